@@ -1,0 +1,7 @@
+// constants.h -- unit and cosmology constants of the reference
+// (src/Cosmoinput3D.h:2-17), defined in cosmo.cpp.
+#pragma once
+namespace ifdm {
+extern const double m2kpc, m2Mpc, cm2Mpc, s2Myr, kg2eV, kg2solar, rhoc, h_0, H_0, hbar, Omega_m0, bigG, c_light;
+double scale_integral(double a0, double a1, int n, double epsrel, int limit);
+}

@@ -1,0 +1,159 @@
+// fft_core.h -- device-side building blocks of the pencil FFT passes for
+// gfx950: complex arithmetic, radix-2/4/8 butterflies and a mixed-radix
+// Stockham line transform whose first and last stages work on registers.
+//
+// A line of N points is transformed by P = N/E threads (E = 8 points per
+// thread).  Thread t holds the points at positions t + e*P, e = 0..E-1
+// ("natural register order") on entry and holds the transform at the same
+// positions on exit.  Input and output therefore have the same coalesced
+// pattern in global memory (consecutive threads, consecutive points), and a
+// forward transform can be followed by an elementwise multiply and an inverse
+// transform without any exchange between them.  Intermediate stages exchange
+// through LDS (one write + one read per stage boundary).
+//
+// Stockham autosort, stage s with radix R and stride Ns = product of the
+// earlier radices: butterfly j reads x[j + r*N/R], multiplies point r by
+// W_N^(r*(j mod Ns)*N/(Ns*R)), takes an R-point DFT and writes
+// y[(j/Ns)*Ns*R + (j mod Ns) + r*Ns].
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ifdm {
+
+template <typename T> struct cx;
+template <> struct __attribute__((aligned(16))) cx<double> { double x, y; };
+template <> struct __attribute__((aligned(8))) cx<float> { float x, y; };
+
+template <typename T> __device__ __forceinline__ cx<T> mkc(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
+template <typename T> __device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return mkc<T>(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return mkc<T>(a.x - b.x, a.y - b.y); }
+template <typename T> __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+    return mkc<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T> __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return mkc<T>(a.x * s, a.y * s); }
+// a * (DIR * i)
+template <int DIR, typename T> __device__ __forceinline__ cx<T> crot(cx<T> a) {
+    return DIR < 0 ? mkc<T>(a.y, -a.x) : mkc<T>(-a.y, a.x);
+}
+
+// ---------------------------------------------------------------- schedules
+// Radix per stage (first stage first); E = points per thread.
+template <int N> struct Sched;
+#define IFDM_SCHED(NN, NSTG, R0, R1, R2, R3)                               \
+    template <> struct Sched<NN> {                                         \
+        static constexpr int ns = NSTG;                                    \
+        static constexpr int E = 8;                                        \
+        static constexpr int r[4] = {R0, R1, R2, R3};                      \
+    };
+IFDM_SCHED(8, 1, 8, 1, 1, 1)
+IFDM_SCHED(16, 2, 8, 2, 1, 1)
+IFDM_SCHED(32, 2, 8, 4, 1, 1)
+IFDM_SCHED(64, 2, 8, 8, 1, 1)
+IFDM_SCHED(128, 3, 8, 8, 2, 1)
+IFDM_SCHED(256, 3, 8, 8, 4, 1)
+IFDM_SCHED(512, 3, 8, 8, 8, 1)
+IFDM_SCHED(1024, 4, 8, 8, 8, 2)
+#undef IFDM_SCHED
+
+template <int N> constexpr int ns_before(int s) {
+    int p = 1;
+    for (int i = 0; i < s; ++i) p *= Sched<N>::r[i];
+    return p;
+}
+
+// LDS position of point p of a line: one pad slot per 8 points.
+__device__ __forceinline__ int lpos(int p) { return p + (p >> 3); }
+// LDS line stride in elements: congruent to 2 mod 16 so that lines of a
+// column tile fall on different banks.
+template <int N> constexpr int lds_line() { return ((N + N / 8 + 15) / 16) * 16 + 2; }
+
+// ---------------------------------------------------------------- butterflies
+template <int DIR, typename T> __device__ __forceinline__ void bfly2(cx<T> &a0, cx<T> &a1) {
+    cx<T> t = a0;
+    a0 = cadd(t, a1);
+    a1 = csub(t, a1);
+}
+template <int DIR, typename T> __device__ __forceinline__ void bfly4(cx<T> &a0, cx<T> &a1, cx<T> &a2, cx<T> &a3) {
+    cx<T> b0 = cadd(a0, a2), b1 = csub(a0, a2), b2 = cadd(a1, a3), b3 = crot<DIR>(csub(a1, a3));
+    a0 = cadd(b0, b2);
+    a2 = csub(b0, b2);
+    a1 = cadd(b1, b3);
+    a3 = csub(b1, b3);
+}
+template <int DIR, typename T> __device__ __forceinline__ void bfly8(cx<T> *a) {
+    bfly4<DIR>(a[0], a[2], a[4], a[6]);
+    bfly4<DIR>(a[1], a[3], a[5], a[7]);
+    const T h = (T)0.707106781186547524400844362104849;
+    cx<T> o1 = a[3], o3 = a[7];
+    a[3] = mkc<T>(h * (o1.x - DIR * o1.y), h * (o1.y + DIR * o1.x));     // * W8^1
+    a[5] = crot<DIR>(a[5]);                                               // * W8^2
+    a[7] = mkc<T>(h * (-o3.x - DIR * o3.y), h * (-o3.y + DIR * o3.x));   // * W8^3
+    cx<T> e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+    cx<T> p0 = a[1], p1 = a[3], p2 = a[5], p3 = a[7];
+    a[0] = cadd(e0, p0); a[4] = csub(e0, p0);
+    a[1] = cadd(e1, p1); a[5] = csub(e1, p1);
+    a[2] = cadd(e2, p2); a[6] = csub(e2, p2);
+    a[3] = cadd(e3, p3); a[7] = csub(e3, p3);
+}
+template <int R, int DIR, typename T> __device__ __forceinline__ void dft(cx<T> *a) {
+    if constexpr (R == 2) bfly2<DIR>(a[0], a[1]);
+    else if constexpr (R == 4) bfly4<DIR>(a[0], a[1], a[2], a[3]);
+    else if constexpr (R == 8) bfly8<DIR>(a);
+}
+
+// ---------------------------------------------------------------- line FFT
+// One Stockham stage S of an N-point transform, direction DIR (-1 forward,
+// +1 inverse), for the thread at line position t.  L is the line's LDS
+// region; tw the table W_N^k = exp(-2 pi i k / N), k < N.
+template <typename T, int N, int DIR, int S>
+__device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const cx<T> *__restrict__ tw) {
+    constexpr int E = Sched<N>::E, P = N / E, R = Sched<N>::r[S], Q = E / R;
+    constexpr int NS = ns_before<N>(S);
+    constexpr bool FIRST = (S == 0), LAST = (S == Sched<N>::ns - 1);
+    cx<T> a[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int j = t + q * P;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if constexpr (FIRST) a[q][r] = v[q + r * Q];
+            else a[q][r] = L[lpos(j + r * (N / R))];
+        }
+        if constexpr (!FIRST) {
+            const int k = j & (NS - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cx<T> w = tw[r * k * (N / (NS * R))];
+                if constexpr (DIR > 0) w.y = -w.y;
+                a[q][r] = cmul(a[q][r], w);
+            }
+        }
+        dft<R, DIR>(a[q]);
+    }
+    if constexpr (LAST) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q + r * Q] = a[q][r];
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int j = t + q * P;
+            const int base = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+            for (int r = 0; r < R; ++r) L[lpos(base + r * NS)] = a[q][r];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int N, int DIR>
+__device__ __forceinline__ void fft_line(cx<T> *v, cx<T> *L, int t, const cx<T> *__restrict__ tw) {
+    fft_stage<T, N, DIR, 0>(v, L, t, tw);
+    if constexpr (Sched<N>::ns > 1) fft_stage<T, N, DIR, 1>(v, L, t, tw);
+    if constexpr (Sched<N>::ns > 2) fft_stage<T, N, DIR, 2>(v, L, t, tw);
+    if constexpr (Sched<N>::ns > 3) fft_stage<T, N, DIR, 3>(v, L, t, tw);
+}
+
+}  // namespace ifdm

@@ -1,0 +1,526 @@
+// ifdm_capi.hip -- the C ABI of libifdm.so (declared in include/ifdm.h):
+// context set-up, device buffers, the launch sequence of the fused passes and
+// the host-side step loop.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ifdm.h"
+#include "constants.h"
+#include "kdk_passes.h"
+
+using namespace ifdm;
+
+struct ifdm_ctx {
+    ifdm_config cfg{};
+    int n = 0;
+    long n3 = 0;
+    int hs = 0;
+    hipStream_t stream = nullptr;
+    void *psi = nullptr;       // N^3 complex
+    void *phi = nullptr;       // N^3 real
+    void *shat = nullptr;      // N*N*HS complex
+    void *tw = nullptr;        // N complex twiddles
+    double *k2 = nullptr;      // N doubles
+    double *part_rho = nullptr;
+    double *part_phi = nullptr;
+    double *red = nullptr;     // 3 doubles on device
+    double *red_host = nullptr;  // 3 doubles, pinned
+    int nz_blocks = 0;
+    // physics
+    double m_over_hbar = 0;    // (mfdm / c^2) / hbar
+    double g_over_hbar = 0;
+    double ms = 0;             // mfdm_solar
+    double ke_coef = 0;
+    double green = 0;
+    double fftnorm = 0;
+    // optional live per-pass timing (ifdm_set_profiling)
+    bool prof = false;
+    hipEvent_t ev[IFDM_NPASS + 1] = {};
+    double pass_ms[IFDM_NPASS] = {};
+    long pass_count = 0;
+    std::string err;
+};
+
+static thread_local std::string g_create_err;
+
+#define HIPCHK(ctx, expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                \
+            return IFDM_E_HIP;                                                             \
+        }                                                                                  \
+    } while (0)
+
+#define LAUNCHCHK(ctx) HIPCHK(ctx, hipGetLastError())
+
+namespace {
+
+template <typename T, int N>
+struct Ops {
+    static constexpr int E = Sched<N>::E, P = N / E;
+
+    static ZArgs zargs(ifdm_ctx *c, double aphi, double arho) {
+        ZArgs z;
+        z.aphi = aphi;
+        z.arho = arho;
+        z.ms = c->ms;
+        z.rhoc = c->cfg.rhoc;
+        z.ref_range = c->cfg.ref_maxphi_range;
+        return z;
+    }
+    static SArgs sargs(ifdm_ctx *c, double dKE) {
+        SArgs s;
+        s.k2 = c->k2;
+        s.ke_coef = c->ke_coef;
+        s.dKE = dKE;
+        s.green = c->green;
+        s.fftnorm = c->fftnorm;
+        return s;
+    }
+
+    template <int MODE>
+    static int zpass(ifdm_ctx *c, const ZArgs &za, double *part) {
+        const int nb = (N * N + z_lines<N>() - 1) / z_lines<N>();
+        hipLaunchKernelGGL((k_zpass<T, N, MODE>), dim3(nb), dim3(z_lines<N>() * P), 0, c->stream,
+                           (cx<T> *)c->psi, (T *)c->phi, (cx<T> *)c->shat, (const cx<T> *)c->tw, za, part);
+        LAUNCHCHK(c);
+        return 0;
+    }
+    template <int AXIS, int MODE>
+    static int spass(ifdm_ctx *c, void *arr, int rl, int kv, const SArgs &sa) {
+        constexpr int TC = s_cols<T>();
+        dim3 grid((kv + TC - 1) / TC, N);
+        hipLaunchKernelGGL((k_spass<T, N, AXIS, MODE>), grid, dim3(TC * P), 0, c->stream,
+                           (cx<T> *)arr, rl, kv, (const cx<T> *)c->tw, sa);
+        LAUNCHCHK(c);
+        return 0;
+    }
+
+    // One pass of the step by index: 0..8 = A..I, 9 = final reduction.
+    // (aphi, arho) is the kick of pass A (both 0: no kick); dKE the drift of C.
+    static int pass(ifdm_ctx *c, int idx, double aphi, double arho, double dKE) {
+        const SArgs sa = sargs(c, dKE);
+        switch (idx) {
+            case 0:
+                if (aphi != 0.0 || arho != 0.0) return zpass<Z_KICK_FWD>(c, zargs(c, aphi, arho), nullptr);
+                return zpass<Z_FWD>(c, zargs(c, 0, 0), nullptr);
+            case 1: return spass<1, S_FWD>(c, c->psi, N, N, sa);
+            case 2: return spass<0, S_DRIFT>(c, c->psi, N, N, sa);
+            case 3: return spass<1, S_INV>(c, c->psi, N, N, sa);
+            case 4: return zpass<Z_INV_SRC>(c, zargs(c, 0, 0), c->part_rho);
+            case 5: return spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sa);
+            case 6: return spass<0, S_GREEN>(c, c->shat, c->hs, N / 2 + 1, sa);
+            case 7: return spass<1, S_INV>(c, c->shat, c->hs, N / 2 + 1, sa);
+            case 8: return zpass<Z_C2R>(c, zargs(c, 0, 0), c->part_phi);
+            case 9:
+                hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, c->stream, c->part_rho, c->nz_blocks,
+                                   c->part_phi, c->nz_blocks, c->red);
+                LAUNCHCHK(c);
+                return 0;
+            default: return IFDM_E_ARG;
+        }
+    }
+
+    static int poisson(ifdm_ctx *c) {
+        int rc;
+        if ((rc = zpass<Z_SRC>(c, zargs(c, 0, 0), c->part_rho))) return rc;
+        for (int i = 5; i <= 9; ++i)
+            if ((rc = pass(c, i, 0, 0, 0))) return rc;
+        return 0;
+    }
+
+    // passes A..I + reduction: kick (aphi, arho), drift dKE, Poisson.
+    // With profiling on, each pass is bracketed by events on the stream.
+    static int kick_drift_poisson(ifdm_ctx *c, double aphi, double arho, double dKE) {
+        int rc;
+        for (int i = 0; i < IFDM_NPASS; ++i) {
+            if (c->prof) HIPCHK(c, hipEventRecord(c->ev[i], c->stream));
+            if ((rc = pass(c, i, aphi, arho, dKE))) return rc;
+        }
+        if (c->prof) HIPCHK(c, hipEventRecord(c->ev[IFDM_NPASS], c->stream));
+        return 0;
+    }
+
+    static int drift(ifdm_ctx *c, double dKE) {
+        SArgs sa = sargs(c, dKE);
+        int rc;
+        if ((rc = zpass<Z_FWD>(c, zargs(c, 0, 0), nullptr))) return rc;
+        if ((rc = spass<1, S_FWD>(c, c->psi, N, N, sa))) return rc;
+        if ((rc = spass<0, S_DRIFT>(c, c->psi, N, N, sa))) return rc;
+        if ((rc = spass<1, S_INV>(c, c->psi, N, N, sa))) return rc;
+        return zpass<Z_INV>(c, zargs(c, 0, 0), nullptr);
+    }
+
+    static int fft3d(ifdm_ctx *c, int sign) {
+        SArgs sa = sargs(c, 0.0);
+        int rc;
+        if (sign < 0) {
+            if ((rc = zpass<Z_FWD>(c, zargs(c, 0, 0), nullptr))) return rc;
+            if ((rc = spass<1, S_FWD>(c, c->psi, N, N, sa))) return rc;
+            return spass<0, S_FWD>(c, c->psi, N, N, sa);
+        }
+        if ((rc = zpass<Z_INV>(c, zargs(c, 0, 0), nullptr))) return rc;
+        if ((rc = spass<1, S_INV>(c, c->psi, N, N, sa))) return rc;
+        return spass<0, S_INV>(c, c->psi, N, N, sa);
+    }
+
+    static int kick(ifdm_ctx *c, double aphi, double arho) {
+        long nb = (c->n3 + 255) / 256;
+        if (nb > 65536) nb = 65536;
+        hipLaunchKernelGGL((k_kick<T>), dim3((unsigned)nb), dim3(256), 0, c->stream, (cx<T> *)c->psi,
+                           (const T *)c->phi, c->n3, aphi, arho);
+        LAUNCHCHK(c);
+        return 0;
+    }
+};
+
+struct Dispatch {
+    int (*pass)(ifdm_ctx *, int, double, double, double);
+    int (*poisson)(ifdm_ctx *);
+    int (*kdp)(ifdm_ctx *, double, double, double);
+    int (*drift)(ifdm_ctx *, double);
+    int (*fft3d)(ifdm_ctx *, int);
+    int (*kick)(ifdm_ctx *, double, double);
+};
+
+template <typename T, int N>
+Dispatch make_dispatch() {
+    return Dispatch{&Ops<T, N>::pass, &Ops<T, N>::poisson, &Ops<T, N>::kick_drift_poisson, &Ops<T, N>::drift, &Ops<T, N>::fft3d,
+                    &Ops<T, N>::kick};
+}
+
+bool get_dispatch(int n, int precision, Dispatch *d) {
+    if (precision != IFDM_FP64) return false;
+    switch (n) {
+        case 8: *d = make_dispatch<double, 8>(); return true;
+        case 16: *d = make_dispatch<double, 16>(); return true;
+        case 32: *d = make_dispatch<double, 32>(); return true;
+        case 64: *d = make_dispatch<double, 64>(); return true;
+        case 128: *d = make_dispatch<double, 128>(); return true;
+        case 256: *d = make_dispatch<double, 256>(); return true;
+        case 512: *d = make_dispatch<double, 512>(); return true;
+        case 1024: *d = make_dispatch<double, 1024>(); return true;
+        default: return false;
+    }
+}
+
+// adds the elapsed time of each pass of the last recorded step
+int accumulate_profile(ifdm_ctx *c) {
+    if (!c->prof) return 0;
+    HIPCHK(c, hipEventSynchronize(c->ev[IFDM_NPASS]));
+    for (int p = 0; p < IFDM_NPASS; ++p) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[p], c->ev[p + 1]));
+        c->pass_ms[p] += ms;
+    }
+    c->pass_count++;
+    return 0;
+}
+
+int read_stats(ifdm_ctx *c, ifdm_stats *st) {
+    HIPCHK(c, hipMemcpyAsync(c->red_host, c->red, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (st) {
+        st->max_phi = c->red_host[0];
+        st->max_rho = c->red_host[1];
+        st->norm = c->red_host[2];
+    }
+    return 0;
+}
+
+// k-vector of src/struct3D.h:76-85
+double kvec(int i, int n, double xmax) {
+    if (i < n / 2) return i * M_PI / xmax;
+    return (static_cast<double>(i) - n) * M_PI / xmax;
+}
+
+void release(ifdm_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->cfg.device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &e : c->ev)
+        if (e) hipEventDestroy(e);
+    void *bufs[] = {c->psi, c->phi, c->shat, c->tw, c->k2, c->part_rho, c->part_phi, c->red};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    if (c->red_host) hipHostFree(c->red_host);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+bool pow2_in_range(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+const char *ifdm_create_error(void) { return g_create_err.c_str(); }
+
+const char *ifdm_last_error(const ifdm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
+    g_create_err.clear();
+    if (!cfg || !out) { g_create_err = "null argument"; return IFDM_E_ARG; }
+    *out = nullptr;
+    if (!pow2_in_range(cfg->n)) { g_create_err = "n must be a power of two in [8, 1024]"; return IFDM_E_ARG; }
+    if (cfg->precision != IFDM_FP64) { g_create_err = "precision mode not available"; return IFDM_E_ARG; }
+    if (!(cfg->xmax > 0) || !(cfg->mfdm > 0)) { g_create_err = "xmax and mfdm must be positive"; return IFDM_E_ARG; }
+    Dispatch d;
+    if (!get_dispatch(cfg->n, cfg->precision, &d)) { g_create_err = "unsupported size"; return IFDM_E_ARG; }
+
+    ifdm_ctx *c = new ifdm_ctx();
+    c->cfg = *cfg;
+    c->n = cfg->n;
+    c->n3 = (long)cfg->n * cfg->n * cfg->n;
+    c->hs = hs_of(cfg->n);
+    const int n = c->n;
+    auto fail = [&](int code, const std::string &m) {
+        g_create_err = m;
+        release(c);
+        return code;
+    };
+    hipError_t e = hipSetDevice(cfg->device);
+    if (e != hipSuccess) return fail(IFDM_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail(IFDM_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    const size_t rs = sizeof(double), cs = 2 * sizeof(double);
+    // z-pass block count: lines per block = min(256 / (n / 8), n * n)
+    const int per = std::min(256 / (n / 8), n * n);
+    c->nz_blocks = (n * n + per - 1) / per;
+    struct { void **p; size_t bytes; } allocs[] = {
+        {&c->psi, cs * (size_t)c->n3},
+        {&c->phi, rs * (size_t)c->n3},
+        {&c->shat, cs * (size_t)n * n * c->hs},
+        {&c->tw, cs * (size_t)n},
+        {(void **)&c->k2, rs * (size_t)n},
+        {(void **)&c->part_rho, 2 * rs * (size_t)c->nz_blocks},
+        {(void **)&c->part_phi, rs * (size_t)c->nz_blocks},
+        {(void **)&c->red, 3 * rs},
+    };
+    for (auto &a : allocs) {
+        if ((e = hipMalloc(a.p, a.bytes)) != hipSuccess)
+            return fail(IFDM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    if ((e = hipHostMalloc((void **)&c->red_host, 3 * rs)) != hipSuccess)
+        return fail(IFDM_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    // twiddles W_N^k = exp(-2 pi i k / N) evaluated in long double
+    std::vector<double> tw(2 * (size_t)n), k2(n);
+    for (int k = 0; k < n; ++k) {
+        const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+        tw[2 * k] = (double)cosl(a);
+        tw[2 * k + 1] = (double)-sinl(a);
+        k2[k] = std::pow(kvec(k, n, cfg->xmax), 2);
+    }
+    if ((e = hipMemcpy(c->tw, tw.data(), cs * n, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(c->k2, k2.data(), rs * n, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemset(c->psi, 0, cs * (size_t)c->n3)) != hipSuccess ||
+        (e = hipMemset(c->phi, 0, rs * (size_t)c->n3)) != hipSuccess ||
+        (e = hipMemset(c->shat, 0, cs * (size_t)n * n * c->hs)) != hipSuccess)
+        return fail(IFDM_E_HIP, std::string("init copy: ") + hipGetErrorString(e));
+    const double mt = cfg->mfdm / std::pow(c_light, 2);
+    c->m_over_hbar = mt / hbar;
+    c->g_over_hbar = cfg->g / hbar;
+    c->ms = cfg->mfdm * kg2eV / kg2solar;                   // src/CosmoBEC_MPI_stan.cpp:330
+    c->ke_coef = 0.5 * hbar / mt;                            // src/struct3D.h:121
+    c->green = -4.0 * M_PI * bigG;                           // src/CosmoBEC_MPI_stan.cpp:105
+    c->fftnorm = 1. / static_cast<double>(std::pow(n, 3));   // src/struct3D.h:69
+    *out = c;
+    return IFDM_OK;
+}
+
+void ifdm_destroy(ifdm_ctx *ctx) { release(ctx); }
+
+int ifdm_set_psi(ifdm_ctx *c, const double *psi) {
+    if (!c || !psi) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    HIPCHK(c, hipMemcpyAsync(c->psi, psi, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
+}
+
+int ifdm_get_psi(ifdm_ctx *c, double *psi) {
+    if (!c || !psi) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    HIPCHK(c, hipMemcpyAsync(psi, c->psi, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
+}
+
+int ifdm_get_phi(ifdm_ctx *c, double *phi) {
+    if (!c || !phi) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    HIPCHK(c, hipMemcpyAsync(phi, c->phi, sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
+}
+
+int ifdm_set_phi(ifdm_ctx *c, const double *phi) {
+    if (!c || !phi) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    HIPCHK(c, hipMemcpyAsync(c->phi, phi, sizeof(double) * (size_t)c->n3, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
+}
+
+int ifdm_poisson(ifdm_ctx *c, ifdm_stats *st) {
+    if (!c) return IFDM_E_ARG;
+    Dispatch d;
+    get_dispatch(c->n, c->cfg.precision, &d);
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    int rc = d.poisson(c);
+    if (rc) return rc;
+    return read_stats(c, st);
+}
+
+int ifdm_kick(ifdm_ctx *c, double dPE, double dQI) {
+    if (!c) return IFDM_E_ARG;
+    Dispatch d;
+    get_dispatch(c->n, c->cfg.precision, &d);
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    int rc = d.kick(c, 0.5 * dPE * c->m_over_hbar, 0.5 * dQI * c->g_over_hbar);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
+}
+
+int ifdm_drift(ifdm_ctx *c, double dKE) {
+    if (!c) return IFDM_E_ARG;
+    Dispatch d;
+    get_dispatch(c->n, c->cfg.precision, &d);
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    int rc = d.drift(c, dKE);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
+}
+
+int ifdm_fft3d(ifdm_ctx *c, int sign) {
+    if (!c || (sign != -1 && sign != 1)) return IFDM_E_ARG;
+    Dispatch d;
+    get_dispatch(c->n, c->cfg.precision, &d);
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    int rc = d.fft3d(c, sign);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
+}
+
+int ifdm_kdk_step(ifdm_ctx *c, const ifdm_dt *dt, ifdm_stats *st) {
+    if (!c || !dt) return IFDM_E_ARG;
+    Dispatch d;
+    get_dispatch(c->n, c->cfg.precision, &d);
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const double aphi = 0.5 * dt->dPE * c->m_over_hbar, arho = 0.5 * dt->dQI * c->g_over_hbar;
+    int rc = d.kdp(c, aphi, arho, dt->dKE);
+    if (rc) return rc;
+    if ((rc = d.kick(c, aphi, arho))) return rc;
+    return read_stats(c, st);
+}
+
+int ifdm_run(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, ifdm_stats *st) {
+    if (!c || (!sched && nsteps > 0) || nsteps < 0) return IFDM_E_ARG;
+    Dispatch d;
+    get_dispatch(c->n, c->cfg.precision, &d);
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    double pphi = 0.0, prho = 0.0;  // pending second half-kick of the previous step
+    int rc;
+    for (int s = 0; s < nsteps; ++s) {
+        const double aphi = 0.5 * sched[s].dPE * c->m_over_hbar, arho = 0.5 * sched[s].dQI * c->g_over_hbar;
+        if ((rc = d.kdp(c, pphi + aphi, prho + arho, sched[s].dKE))) return rc;
+        pphi = aphi;
+        prho = arho;
+        if (st && (rc = read_stats(c, &st[s]))) return rc;
+        if ((rc = accumulate_profile(c))) return rc;
+    }
+    if (nsteps > 0 && (rc = d.kick(c, pphi, prho))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
+}
+
+int ifdm_simulate(ifdm_ctx *c, double a0, int nsteps, double *a_out, ifdm_dt *sched_out) {
+    if (!c || nsteps < 0 || !(a0 > 0.0)) return IFDM_E_ARG;
+    Dispatch d;
+    get_dispatch(c->n, c->cfg.precision, &d);
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    int rc;
+    if ((rc = d.poisson(c))) return rc;                            // :218 of the first step
+    ifdm_stats st;
+    if ((rc = read_stats(c, &st))) return rc;
+    double a = a0, pphi = 0.0, prho = 0.0;
+    for (int s = 0; s < nsteps; ++s) {
+        ifdm_dt dt;
+        double an;
+        if ((rc = ifdm_timestep_criteria(&c->cfg, a, &st, &dt, &an, nullptr))) {
+            c->err = "timestep criteria failed";
+            return rc;
+        }
+        if (sched_out) sched_out[s] = dt;
+        a = an;
+        const double aphi = 0.5 * dt.dPE * c->m_over_hbar, arho = 0.5 * dt.dQI * c->g_over_hbar;
+        if ((rc = d.kdp(c, pphi + aphi, prho + arho, dt.dKE))) return rc;
+        pphi = aphi;
+        prho = arho;
+        if ((rc = read_stats(c, &st))) return rc;
+        if ((rc = accumulate_profile(c))) return rc;
+    }
+    if (nsteps > 0 && (rc = d.kick(c, pphi, prho))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (a_out) *a_out = a;
+    return IFDM_OK;
+}
+
+int ifdm_set_profiling(ifdm_ctx *c, int on) {
+    if (!c) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (on && !c->ev[0])
+        for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
+    c->prof = on != 0;
+    for (double &x : c->pass_ms) x = 0.0;
+    c->pass_count = 0;
+    return IFDM_OK;
+}
+
+int ifdm_get_profile(const ifdm_ctx *c, double *ms_sum, long *count) {
+    if (!c || !ms_sum || !count) return IFDM_E_ARG;
+    for (int p = 0; p < IFDM_NPASS; ++p) ms_sum[p] = c->pass_ms[p];
+    *count = c->pass_count;
+    return IFDM_OK;
+}
+
+int ifdm_time_passes(ifdm_ctx *c, const ifdm_dt *dt, int iters, float *ms_out) {
+    if (!c || !dt || iters <= 0 || !ms_out) return IFDM_E_ARG;
+    Dispatch d;
+    get_dispatch(c->n, c->cfg.precision, &d);
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const double aphi = 0.5 * dt->dPE * c->m_over_hbar, arho = 0.5 * dt->dQI * c->g_over_hbar;
+    hipEvent_t ev[IFDM_NPASS + 1];
+    for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+    std::vector<double> acc(IFDM_NPASS, 0.0);
+    int rc = 0;
+    for (int it = 0; it < iters && !rc; ++it) {
+        for (int p = 0; p < IFDM_NPASS && !rc; ++p) {
+            if (hipEventRecord(ev[p], c->stream) != hipSuccess) { rc = IFDM_E_HIP; break; }
+            rc = d.pass(c, p, 2 * aphi, 2 * arho, dt->dKE);
+            if (!rc && hipEventRecord(ev[p + 1], c->stream) != hipSuccess) rc = IFDM_E_HIP;
+        }
+        if (!rc && hipEventSynchronize(ev[IFDM_NPASS]) != hipSuccess) rc = IFDM_E_HIP;
+        for (int p = 0; p < IFDM_NPASS && !rc; ++p) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[p], ev[p + 1]) != hipSuccess) { rc = IFDM_E_HIP; break; }
+            acc[p] += ms;
+        }
+    }
+    for (auto &e : ev) hipEventDestroy(e);
+    if (rc) { if (c->err.empty()) c->err = "event timing failed"; return rc; }
+    for (int p = 0; p < IFDM_NPASS; ++p) ms_out[p] = (float)(acc[p] / iters);
+    return IFDM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,248 @@
+// kdk_passes.h -- the fused pencil passes of one kick-drift-kick step.
+//
+// Fields in HBM (C order, z fastest, as the reference's opr.wfc):
+//   psi   N^3 complex                 (in-place spectral work)
+//   phi   N^3 real, unpadded
+//   shat  N * N * HS complex: half spectrum of the Poisson source, z extent
+//         N/2+1 valid entries padded to HS (a multiple of 8) so that each
+//         row starts on a 128-byte boundary.
+//
+// One step is nine passes (A..I), each one read and one write of a field:
+//   A  z:  kick (phase from phi and |psi|^2), forward FFT            psi
+//   B  y:  forward FFT                                                psi
+//   C  x:  forward FFT, drift exp(-i KE dKE)/N^3, inverse FFT         psi
+//   D  y:  inverse FFT                                                psi
+//   E  z:  inverse FFT -> psi; m_s|psi|^2 - rho_c -> real FFT -> shat;
+//          block max |psi|^2 and sum |psi|^2
+//   F  y:  forward FFT                                                shat
+//   G  x:  forward FFT, Green -4 pi G / k^2 / N^3 (k = 0 -> 0), inverse shat
+//   H  y:  inverse FFT                                                shat
+//   I  z:  complex-to-real inverse FFT -> phi; block max |phi|
+// The reference's order is src/CosmoBEC_MPI_stan.cpp:130-152 with
+// poisson() at :80-119; its drift multiplies by fftnorm at :141 and its
+// Poisson solve by fftnorm after the c2r at :116.
+#pragma once
+#include "fft_core.h"
+
+namespace ifdm {
+
+// half-spectrum row length
+__host__ __device__ constexpr int hs_of(int n) { return ((n / 2 + 1 + 7) / 8) * 8; }
+
+struct ZArgs {
+    double aphi;      // kick: phase coefficient of phi
+    double arho;      // kick: phase coefficient of |psi|^2
+    double ms;        // Poisson source: m_s (times 2^(2s) for a scaled psi)
+    double rhoc;      // Poisson source: subtracted mean density
+    int ref_range;    // max|phi| over the reference's one-rank scan range
+};
+
+struct SArgs {
+    const double *k2;  // k_i^2, i < N (src/struct3D.h:76-85 squared)
+    double ke_coef;    // 0.5 hbar / (m / c^2)   (src/struct3D.h:121)
+    double dKE;        // time integral of the drift
+    double green;      // -4 pi G
+    double fftnorm;    // 1 / N^3
+};
+
+enum ZMode { Z_FWD = 0, Z_INV = 1, Z_KICK_FWD = 2, Z_INV_SRC = 3, Z_SRC = 4, Z_C2R = 5 };
+enum SMode { S_FWD = 0, S_INV = 1, S_DRIFT = 2, S_GREEN = 3 };
+
+template <int N> constexpr int z_lines() {
+    constexpr int P = N / Sched<N>::E;
+    return (256 / P) < (N * N) ? (256 / P) : (N * N);
+}
+template <typename T> constexpr int s_cols() { return 128 / (int)sizeof(cx<T>); }  // 128-byte rows
+
+// block reduction over the whole work-group; sbuf holds >= nthreads doubles
+template <bool MAX>
+__device__ __forceinline__ double block_reduce(double v, double *sbuf, int tid, int nthreads) {
+    __syncthreads();
+    sbuf[tid] = v;
+    __syncthreads();
+    for (int s = nthreads / 2; s > 0; s >>= 1) {
+        if (tid < s) sbuf[tid] = MAX ? fmax(sbuf[tid], sbuf[tid + s]) : sbuf[tid] + sbuf[tid + s];
+        __syncthreads();
+    }
+    return sbuf[0];
+}
+
+// ------------------------------------------------------------------ z passes
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__(z_lines<N>() * (N / Sched<N>::E))
+void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ shat,
+             const cx<T> *__restrict__ tw, ZArgs za, double *__restrict__ part) {
+    constexpr int E = Sched<N>::E, P = N / E, LZ = z_lines<N>(), LS = lds_line<N>();
+    constexpr int HS = hs_of(N), NT = LZ * P;
+    __shared__ cx<T> lds[LZ * LS];
+    const int tid = threadIdx.x, l = tid / P, t = tid - l * P;
+    const long line = (long)blockIdx.x * LZ + l;
+    const bool valid = line < (long)N * N;
+    cx<T> *L = lds + l * LS;
+    cx<T> v[E];
+
+    if constexpr (MODE == Z_C2R) {
+        const cx<T> *row = shat + line * HS;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int p = t + e * P;
+            cx<T> s = mkc<T>(0, 0);
+            if (valid) {
+                if (p <= N / 2) s = row[p];
+                else { s = row[N - p]; s.y = -s.y; }
+            }
+            v[e] = s;
+        }
+        fft_line<T, N, +1>(v, L, t, tw);
+        double mx = 0.0;
+        T *out = phi + line * N;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int p = t + e * P;
+            if (valid) {
+                out[p] = v[e].x;
+                const bool in = !za.ref_range || (line * (N + 2) + p < (long)N * N * N);
+                if (in) mx = fmax(mx, fabs((double)v[e].x));
+            }
+        }
+        double *sb = reinterpret_cast<double *>(lds);
+        const double bm = block_reduce<true>(mx, sb, tid, NT);
+        if (tid == 0) part[blockIdx.x] = bm;
+        return;
+    } else {
+        cx<T> *row = psi + line * N;
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = valid ? row[t + e * P] : mkc<T>(0, 0);
+
+        if constexpr (MODE == Z_KICK_FWD) {
+            const T *prow = phi + line * N;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const double f = valid ? (double)prow[t + e * P] : 0.0;
+                const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
+                const double th = za.aphi * f + za.arho * rr;
+                double sn, cs;
+                sincos(th, &sn, &cs);
+                v[e] = cmul(v[e], mkc<T>((T)cs, (T)-sn));
+            }
+        }
+        if constexpr (MODE == Z_FWD || MODE == Z_KICK_FWD) {
+            fft_line<T, N, -1>(v, L, t, tw);
+            if (valid) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
+            }
+            return;
+        }
+        if constexpr (MODE == Z_INV || MODE == Z_INV_SRC) {
+            fft_line<T, N, +1>(v, L, t, tw);
+            if (valid) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
+            }
+        }
+        if constexpr (MODE == Z_INV_SRC || MODE == Z_SRC) {
+            double mx = 0.0, sm = 0.0;
+            cx<T> s[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
+                if (valid) { mx = fmax(mx, rr); sm += rr; }
+                s[e] = mkc<T>(valid ? (T)(za.ms * rr - za.rhoc) : (T)0, (T)0);
+            }
+            fft_line<T, N, -1>(s, L, t, tw);
+            if (valid) {
+                cx<T> *srow = shat + line * HS;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int p = t + e * P;
+                    if (p <= N / 2) srow[p] = s[e];
+                }
+            }
+            double *sb = reinterpret_cast<double *>(lds);
+            const double bm = block_reduce<true>(mx, sb, tid, NT);
+            const double bs = block_reduce<false>(sm, sb, tid, NT);
+            if (tid == 0) { part[2 * blockIdx.x] = bm; part[2 * blockIdx.x + 1] = bs; }
+        }
+    }
+}
+
+// ------------------------------------------------------------ strided passes
+// AXIS 1: lines along y (stride rl); AXIS 0: lines along x (stride N*rl).
+// A work-group takes TC consecutive z-columns so each global access of a
+// wavefront covers 128-byte row segments.
+template <typename T, int N, int AXIS, int MODE>
+__global__ __launch_bounds__(s_cols<T>() * (N / Sched<N>::E))
+void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw, SArgs sa) {
+    constexpr int E = Sched<N>::E, P = N / E, TC = s_cols<T>(), LS = lds_line<N>();
+    __shared__ cx<T> lds[TC * LS];
+    const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
+    const int k = blockIdx.x * TC + c;
+    const int other = blockIdx.y;
+    const bool valid = k < kv;
+    long base, stride;
+    if constexpr (AXIS == 1) { base = (long)other * N * rl + k; stride = rl; }
+    else { base = (long)other * rl + k; stride = (long)N * rl; }
+    cx<T> *L = lds + c * LS;
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = valid ? a[base + (long)(t + e * P) * stride] : mkc<T>(0, 0);
+
+    if constexpr (MODE == S_FWD) fft_line<T, N, -1>(v, L, t, tw);
+    else if constexpr (MODE == S_INV) fft_line<T, N, +1>(v, L, t, tw);
+    else {
+        fft_line<T, N, -1>(v, L, t, tw);
+        // spectral indices of this thread's points: along the line i, fixed `other` and k
+        const double k2o = sa.k2[other] + 0.0, k2k = valid ? sa.k2[k] : 0.0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = t + e * P;
+            const double k2 = sa.k2[i] + k2o + k2k;
+            if constexpr (MODE == S_DRIFT) {
+                const double th = -(sa.ke_coef * k2) * sa.dKE;
+                double sn, cs;
+                sincos(th, &sn, &cs);
+                v[e] = cmul(v[e], mkc<T>((T)(cs * sa.fftnorm), (T)(sn * sa.fftnorm)));
+            } else {
+                const double gf = (k2 == 0.0) ? 0.0 : sa.green / k2 * sa.fftnorm;
+                v[e] = cscale(v[e], (T)gf);
+            }
+        }
+        fft_line<T, N, +1>(v, L, t, tw);
+    }
+    if (valid) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) a[base + (long)(t + e * P) * stride] = v[e];
+    }
+}
+
+// --------------------------------------------------------------- elementwise
+template <typename T>
+__global__ __launch_bounds__(256) void k_kick(cx<T> *__restrict__ psi, const T *__restrict__ phi, long n3,
+                                              double aphi, double arho) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
+        cx<T> z = psi[i];
+        const double rr = (double)z.x * z.x + (double)z.y * z.y;
+        const double th = aphi * (double)phi[i] + arho * rr;
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        psi[i] = cmul(z, mkc<T>((T)cs, (T)-sn));
+    }
+}
+
+// final reduction of the per-block partials: out = {max|phi|, max|psi|^2, sum|psi|^2}
+__global__ __launch_bounds__(1024) void k_reduce(const double *__restrict__ prho, int nrho,
+                                                 const double *__restrict__ pphi, int nphi,
+                                                 double *__restrict__ out) {
+    __shared__ double sb[1024];
+    const int tid = threadIdx.x;
+    double mr = 0.0, sm = 0.0, mp = 0.0;
+    for (int i = tid; i < nrho; i += 1024) { mr = fmax(mr, prho[2 * i]); sm += prho[2 * i + 1]; }
+    for (int i = tid; i < nphi; i += 1024) mp = fmax(mp, pphi[i]);
+    const double a = block_reduce<true>(mp, sb, tid, 1024);
+    const double b = block_reduce<true>(mr, sb, tid, 1024);
+    const double c = block_reduce<false>(sm, sb, tid, 1024);
+    if (tid == 0) { out[0] = a; out[1] = b; out[2] = c; }
+}
+
+}  // namespace ifdm

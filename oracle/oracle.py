@@ -1,0 +1,267 @@
+"""CPU oracle for the fuzzy-dark-matter KDK step -- TEST INFRASTRUCTURE ONLY.
+
+This module restates the reference's FP64 algorithm (jchan192/IFDM_Code,
+``src/CosmoBEC_MPI_stan.cpp``) on the CPU.  The array work runs in
+``liboracle.so`` (plain C, ``oracle/ifdm_oracle.c``); the time-step
+criteria and the scale-factor integrals run here, the integrals through
+``scipy.integrate.quad`` -- QUADPACK's QAGS, the routine GSL's
+``gsl_integration_qags`` re-implements.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
+leg may import this module, and only as the checker / baseline.  The product
+(``ifdm_code_amd``) never imports it.
+
+Parity pins (each function -> the independent check in tests/):
+
+=====================  ===================================================
+oracle function        pinned by
+=====================  ===================================================
+fft3d_c2c / r2c / c2r  brute-force O(N^6) DFT at N<=8; numpy pocketfft at
+                       N=16..32; Parseval; impulse -> constant
+poisson                closed form: rho = rho0 (1 + eps cos k0.x) gives
+                       phi = -4 pi G rho0 eps cos(k0.x) / |k0|^2
+kick                   closed form: uniform psi with g != 0 (phase
+                       g|psi|^2 dQI / hbar per two half-kicks); the
+                       potential phase m phi dPE / (c^2 hbar) with the
+                       analytic cosine phi; |psi| unchanged
+drift                  closed form: plane-wave superposition evolves as
+                       A_k exp(-i hbar k^2 dKE / (2 m))
+split_op               norm conservation; time reversal (conjugate, step,
+                       conjugate); Galilean plane wave
+maxima                 brute-force numpy max / sum over the same entries
+timestep_criteria      the kinetic / potential / self-interaction phase of
+                       one step equals 2 pi in the da -> 0 limit (the
+                       criterion's design); integrals vs closed-form
+                       Einstein-de Sitter limit (Omega_m = 1)
+=====================  ===================================================
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+# --- constants: src/Cosmoinput3D.h:2-17 (evaluated with the same libm pow)
+m2kpc = 3.086e19
+m2Mpc = 3.086e22
+cm2Mpc = 3.086e24
+s2Myr = 60.0 * 60.0 * 24.0 * 365.0 * 1e6
+kg2eV = 1.783e-36
+kg2solar = 1.989e30
+rhoc_const = 3.51059e10
+h_0 = 0.677
+H_0 = h_0 * 100 * s2Myr / m2Mpc * 1e3
+hbar = 6.58211928e-16 / s2Myr
+Omega_m0 = 0.276
+bigG = 6.674e-11 * kg2solar / math.pow(m2Mpc, 3) * math.pow(s2Myr, 2)
+c_light = 2.99792458e8 / m2Mpc * s2Myr
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("xmax", ctypes.c_double), ("mfdm", ctypes.c_double),
+                ("mfdm_solar", ctypes.c_double), ("g", ctypes.c_double), ("rhoc", ctypes.c_double),
+                ("hbar", ctypes.c_double), ("c", ctypes.c_double), ("bigG", ctypes.c_double)]
+
+
+def lib():
+    """Load liboracle.so (built by ``make -C oracle``)."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        pp = ctypes.POINTER(_Params)
+        L.oracle_fft3d_c2c.argtypes = [dp, ctypes.c_int, ctypes.c_int]
+        L.oracle_fft3d_r2c.argtypes = [dp, dp, ctypes.c_int]
+        L.oracle_fft3d_c2r.argtypes = [dp, dp, ctypes.c_int]
+        L.oracle_poisson.argtypes = [pp, dp, dp, dp]
+        L.oracle_kick.argtypes = [pp, dp, dp, ctypes.c_double, ctypes.c_double]
+        L.oracle_drift.argtypes = [pp, dp, ctypes.c_double]
+        L.oracle_split_op.argtypes = [pp, dp, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.oracle_maxima.argtypes = [pp, dp, dp, ctypes.c_int, dp]
+        L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@dataclass
+class RunParams:
+    """The run-time parameters of src/CosmoBEC_MPI_stan.cpp:329-339 after
+    its unit conversions (mfdm in eV, g_in in eV cm^3, xmax_in = box/2 in
+    Mpc/h)."""
+    n: int
+    mfdm: float = 2.5e-24
+    g_in: float = 0.0
+    xmax_in: float = 5.0
+    rhoc: float = rhoc_const
+
+    @property
+    def mfdm_solar(self):            # :330
+        return self.mfdm * kg2eV / kg2solar
+
+    @property
+    def g(self):                     # :331
+        return self.g_in / math.pow(cm2Mpc, 3)
+
+    @property
+    def xmax(self):                  # :332
+        return self.xmax_in / h_0
+
+    @property
+    def dx(self):                    # struct3D.h:52
+        return 2.0 * self.xmax / self.n
+
+    def c_params(self) -> _Params:
+        return _Params(self.n, self.xmax, self.mfdm, self.mfdm_solar, self.g, self.rhoc,
+                       hbar, c_light, bigG)
+
+
+def kvec(n: int, xmax: float) -> np.ndarray:
+    """src/struct3D.h:76-85"""
+    i = np.arange(n, dtype=np.float64)
+    return np.where(i < n // 2, i * math.pi / xmax, (i - n) * math.pi / xmax)
+
+
+# ----------------------------------------------------------------- FFTs
+def fft3d_c2c(a: np.ndarray, sign: int) -> np.ndarray:
+    out = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    lib().oracle_fft3d_c2c(_dp(out.view(np.float64)), out.shape[0], int(sign))
+    return out
+
+
+def fft3d_r2c(real: np.ndarray) -> np.ndarray:
+    n = real.shape[0]
+    pad = np.zeros((n, n, 2 * (n // 2 + 1)))
+    pad[:, :, :n] = real
+    out = np.zeros((n, n, n // 2 + 1), np.complex128)
+    lib().oracle_fft3d_r2c(_dp(pad), _dp(out.view(np.float64)), n)
+    return out
+
+
+def fft3d_c2r(half: np.ndarray) -> np.ndarray:
+    n = half.shape[0]
+    h = np.ascontiguousarray(half, dtype=np.complex128).copy()
+    pad = np.zeros((n, n, 2 * (n // 2 + 1)))
+    lib().oracle_fft3d_c2r(_dp(h.view(np.float64)), _dp(pad), n)
+    return pad[:, :, :n].copy()
+
+
+# ----------------------------------------------------------------- state
+class State:
+    """psi (N^3 complex128), phi in the reference's padded layout and the
+    half-spectrum scratch, as the reference's ``Operators`` (struct3D.h:98)."""
+
+    def __init__(self, rp: RunParams, psi: np.ndarray):
+        n = rp.n
+        self.rp = rp
+        self.cp = rp.c_params()
+        self.psi = np.ascontiguousarray(psi, dtype=np.complex128).reshape(n, n, n).copy()
+        self.phi_pad = np.zeros((n, n, 2 * (n // 2 + 1)))
+        self.phikk = np.zeros((n, n, n // 2 + 1), np.complex128)
+
+    @property
+    def phi(self) -> np.ndarray:
+        return self.phi_pad[:, :, :self.rp.n].copy()
+
+    def poisson(self):
+        lib().oracle_poisson(ctypes.byref(self.cp), _dp(self.psi.view(np.float64)), _dp(self.phi_pad),
+                             _dp(self.phikk.view(np.float64)))
+
+    def kick(self, dPE: float, dQI: float):
+        lib().oracle_kick(ctypes.byref(self.cp), _dp(self.psi.view(np.float64)), _dp(self.phi_pad), dPE, dQI)
+
+    def drift(self, dKE: float):
+        lib().oracle_drift(ctypes.byref(self.cp), _dp(self.psi.view(np.float64)), dKE)
+
+    def split_op(self, dKE: float, dPE: float, dQI: float):
+        lib().oracle_split_op(ctypes.byref(self.cp), _dp(self.psi.view(np.float64)), _dp(self.phi_pad),
+                              _dp(self.phikk.view(np.float64)), dKE, dPE, dQI)
+
+    def maxima(self, ref_range: bool = False):
+        out = np.zeros(3)
+        lib().oracle_maxima(ctypes.byref(self.cp), _dp(self.psi.view(np.float64)), _dp(self.phi_pad),
+                            int(ref_range), _dp(out))
+        return {"max_phi": out[0], "max_rho": out[1], "norm": out[2]}
+
+
+# ----------------------------------------------------------- time step
+def _Ha(x, n):
+    """Ha{2,3,4}_preintegrat, src/struct3D.h:127-146"""
+    H = H_0 * math.sqrt(Omega_m0 / math.pow(x, 3) + (1.0 - Omega_m0))
+    return 1. / (H * math.pow(x, n))
+
+
+def integral(a0: float, a1: float, n: int) -> float:
+    """gsl_integration_qags(.., a0, a1, 0, 1e-10, 1000, ..) of 1/(H a^n)
+    (src/CosmoBEC_MPI_stan.cpp:66-68) via QUADPACK's QAGS."""
+    from scipy.integrate import quad
+    val, _err = quad(_Ha, a0, a1, args=(n,), epsabs=0.0, epsrel=1e-10, limit=1000)
+    return val
+
+
+def timestep_criteria(rp: RunParams, a: float, max_phi: float, max_rho: float):
+    """Timestep_criteria, src/CosmoBEC_MPI_stan.cpp:19-78.
+
+    Returns (dKE, dPE, dQI, a_next, which) with which in {3, 2, 4} naming
+    the limiting criterion as the reference's ``n``."""
+    m_tilde = rp.mfdm / math.pow(c_light, 2)
+    dt_KE = 4. / (3. * math.pi) * m_tilde / hbar * math.pow(rp.dx, 2) * math.pow(a, 2)   # :32
+    dt_PE = 2. * math.pi * hbar / m_tilde / max_phi * a                                   # :38
+    g = rp.g
+    dt_QI = 2. * math.pi * hbar / abs(g) / max_rho * math.pow(a, 3) if g != 0 else 0.0     # :41-42
+    max_dt, which = dt_KE, 3                                                              # :49-58
+    if max_dt > dt_PE:
+        max_dt, which = dt_PE, 2
+    if max_dt > dt_QI and g != 0:
+        max_dt, which = dt_QI, 4
+    H = H_0 * math.sqrt(Omega_m0 / math.pow(a, 3) + (1.0 - Omega_m0))                     # :61
+    da = H * a * max_dt
+    a_next = a + da
+    dKE = integral(a, a + da, 3)
+    dPE = integral(a, a + da, 2)
+    dQI = integral(a, a + da, 4)
+    return dKE, dPE, dQI, a_next, which
+
+
+def simulate(rp: RunParams, psi0: np.ndarray, a0: float, nsteps: int, ref_maxphi_range: bool = False,
+             schedule=None):
+    """The time loop of Simulation(), src/CosmoBEC_MPI_stan.cpp:212-304,
+    without its snapshot triggers: poisson -> Timestep_criteria ->
+    Split_op_MPI per step.  With ``schedule`` (list of (dKE, dPE, dQI)) the
+    recorded time steps are replayed instead of recomputed.  Returns the
+    final State and the list of (dKE, dPE, dQI, a_next) used."""
+    st = State(rp, psi0)
+    a = a0
+    used = []
+    for tn in range(nsteps):
+        st.poisson()                                                   # :218
+        if schedule is None:
+            m = st.maxima(ref_maxphi_range)
+            dKE, dPE, dQI, a, _ = timestep_criteria(rp, a, m["max_phi"], m["max_rho"])
+        else:
+            dKE, dPE, dQI = schedule[tn]
+        used.append((dKE, dPE, dQI, a))
+        st.split_op(dKE, dPE, dQI)                                     # :304
+    return st, used
+
+
+def set_threads(t: int):
+    lib().oracle_set_threads(int(t))
+
+
+def num_threads() -> int:
+    return lib().oracle_num_threads()
