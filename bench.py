@@ -91,6 +91,51 @@ def cpu_baseline(n: int, psi: np.ndarray, cfg_g_in: float, xmax_in: float, steps
                       f"restatement (poisson + Timestep_criteria + Split_op_MPI, 6 3-D FFTs/step), OpenMP"}
 
 
+def rel_l2(a, b) -> float:
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def timed_adaptive(F, cfg, psi, warmup: int, steps: int):
+    """warm-up + timed adaptive steps (ifdm_simulate) with live per-pass
+    events; returns (steps/s, avg ms per pass, schedule, final psi, solver a)."""
+    s = F.Solver(cfg)
+    s.set_psi(psi)
+    a, sched_w = s.simulate_native(1.0 / 51, warmup)
+    s.set_profiling(True)
+    t0 = time.perf_counter()
+    a, sched_t = s.simulate_native(a, steps)
+    el = time.perf_counter() - t0
+    ms_sum, cnt = s.get_profile()
+    s.set_profiling(False)
+    avg = [m / max(cnt, 1) for m in ms_sum]
+    return s, steps / el, el, avg, sched_w + sched_t, a
+
+
+def counter_traffic(n: int, precision: str) -> dict:
+    """HBM bytes per dispatch of each pass from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/kdk_<n>_<precision>_hbm_traffic.md),
+    already scaled by the copy-kernel calibration; {} when absent."""
+    path = os.path.join(ROOT, "profiles", f"kdk_{n}_{precision}_hbm_traffic.md")
+    out = {}
+    if os.path.exists(path):
+        txt = open(path).read()
+        if "```traffic" in txt:
+            for line in txt.split("```traffic", 1)[1].split("```", 1)[0].strip().splitlines():
+                name, prec, val = line.split()
+                out[name] = float(val)
+    return out
+
+
+def roofline(n: int, avg: list, sc: int, sr: int, precision: str) -> dict:
+    pb = pass_bytes(n, sc, sr)
+    dom = int(np.argmax(avg[:9]))
+    achieved = pb[dom] / (avg[dom] * 1e-3) / 1e9
+    traffic = counter_traffic(n, precision).get(PASS_NAMES[dom])
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": PASS_NAMES[dom],
+            "kernel_avg_ms": round(avg[dom], 4), "kernel_alg_bytes": pb[dom]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +143,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -122,51 +168,67 @@ def main():
         rms_phi = float(np.sqrt(np.mean(phi * phi)))
         mean_rho = st["norm"] / n ** 3
         del phi
+        copy_gbs = s0.stream_copy(2 << 30, 10)
+    log(f"[bench] stream copy {copy_gbs:.0f} GB/s")
     mt = cfg0.mfdm / F.constant("c") ** 2
     g = 0.01 * mt * rms_phi / mean_rho              # eV Mpc^3
     g_in = g * F.constant("cm2Mpc") ** 3            # eV cm^3, the CLI unit
     cfg = F.Config.from_cli(2.5e-24, g_in, xmax_in, n)
     log(f"[bench] g = {g_in:.4e} eV cm^3")
 
-    s = F.Solver(cfg)
-    s.set_psi(psi)
-    a0 = 1.0 / 51
-    a, _ = s.simulate_native(a0, args.warmup)
-    log(f"[bench] warm-up done, a = {a:.5f}")
-    s.set_profiling(True)
-    t0 = time.perf_counter()
-    a, sched = s.simulate_native(a, args.steps)
-    el = time.perf_counter() - t0
-    ms_sum, cnt = s.get_profile()
-    s.set_profiling(False)
-    log(f"[bench] {args.steps} steps in {el:.3f} s, a = {a:.5f}")
-    steps_per_s = args.steps / el
-
-    avg = [m / max(cnt, 1) for m in ms_sum]
+    s, sps, el, avg, sched, a = timed_adaptive(F, cfg, psi, args.warmup, args.steps)
+    log(f"[bench] FP64: {args.steps} steps in {el:.3f} s ({sps:.2f} steps/s), a = {a:.5f}")
+    psi64 = s.get_psi() if not args.no_mixed else None
+    phi64 = s.get_phi() if not args.no_mixed else None
+    s.close()
     pb = pass_bytes(n)
-    dom = int(np.argmax(avg[:9]))
-    achieved = pb[dom] / (avg[dom] * 1e-3) / 1e9
     b_alg = sum(pb)
-    step_gbs = b_alg * steps_per_s / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": PASS_NAMES[dom],
-            "kernel_avg_ms": round(avg[dom], 4), "kernel_alg_bytes": pb[dom]}
-
+    step_gbs = b_alg * sps / 1e9
     out = {
-        "metric": METRIC, "value": round(steps_per_s, 3), "unit": "KDK steps/s", "n_gpus": 1,
+        "metric": METRIC, "value": round(sps, 3), "unit": "KDK steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded Gaussian-random-field psi (delta rms 0.05, phase rms 1 rad, P(k)~k^-2.5)",
-        "config": {"workload": f"FP64 KDK step at {n}^3, box {2 * xmax_in:g} Mpc/h, z=50, mfdm=2.5e-24 eV, "
-                               f"g={g_in:.3e} eV cm^3 (1% of gravity), adaptive dt",
+        "config": {"workload": f"KDK step at {n}^3, box {2 * xmax_in:g} Mpc/h, z=50, mfdm=2.5e-24 eV, "
+                               f"g={g_in:.3e} eV cm^3 (1% of gravity), adaptive dt, FP64",
                    "n": n, "precision": "fp64"},
-        "roofline": roof,
+        "roofline": roofline(n, avg, 16, 8, "fp64"),
         "step_hbm": {"alg_bytes_per_step": b_alg, "achieved_GBs": round(step_gbs, 1),
                      "frac_of_8TBs": round(step_gbs / HBM_PEAK_GBS, 4),
-                     "cell_updates_per_s": round(steps_per_s * n ** 3, 1)},
+                     "cell_updates_per_s": round(sps * n ** 3, 1),
+                     "stream_copy_GBs": round(copy_gbs, 1)},
         "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avg)},
     }
-    s.close()
+
+    if not args.no_mixed:
+        cfgm = F.Config.from_cli(2.5e-24, g_in, xmax_in, n, precision=F.MIXED)
+        sm, spsm, elm, avgm, _schm, am = timed_adaptive(F, cfgm, psi, args.warmup, args.steps)
+        sm.close()
+        log(f"[bench] mixed: {args.steps} steps in {elm:.3f} s ({spsm:.2f} steps/s)")
+        # error: the mixed path replays the FP64 run's schedule from the same IC
+        with F.Solver(cfgm) as sr:
+            sr.set_psi(psi)
+            sr.poisson()
+            sr.run([tuple(x) for x in sched])
+            psim = sr.get_psi()
+            sr.poisson()
+            phim = sr.get_phi()
+        rho64, rhom = np.abs(psi64) ** 2, np.abs(psim) ** 2
+        w = rho64 / rho64.sum()
+        dphase = np.angle(psim * np.conj(psi64))
+        pbm = pass_bytes(n, 8, 4)
+        out["mixed"] = {
+            "value": round(spsm, 3), "unit": "KDK steps/s", "ms_per_step": round(1e3 * elm / args.steps, 4),
+            "speedup_vs_fp64": round(spsm / sps, 3), "dtype": "c64 storage/FP32 FFT, FP64 phases+sums",
+            "roofline": roofline(n, avgm, 8, 4, "mixed"),
+            "step_hbm_frac_of_8TBs": round(sum(pbm) * spsm / 1e9 / HBM_PEAK_GBS, 4),
+            "error_vs_fp64_after_steps": {
+                "steps": len(sched), "density_rel_l2": rel_l2(rhom, rho64),
+                "phase_rho_weighted_rms_rad": float(np.sqrt(np.sum(w * dphase ** 2))),
+                "potential_rel_l2": rel_l2(phim, phi64)},
+            "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avgm)},
+        }
+        del psi64, psim, rho64, rhom, w, dphase, phim, phi64
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(n, psi, g_in, xmax_in, args.cpu_steps)

@@ -48,7 +48,7 @@ EXPORTS = [
     "ifdm_create", "ifdm_destroy", "ifdm_last_error", "ifdm_create_error", "ifdm_set_psi", "ifdm_get_psi",
     "ifdm_get_phi", "ifdm_set_phi", "ifdm_poisson", "ifdm_kick", "ifdm_kdk_step", "ifdm_run", "ifdm_drift",
     "ifdm_fft3d", "ifdm_simulate", "ifdm_time_passes",
-    "ifdm_set_profiling", "ifdm_get_profile", "ifdm_timestep_criteria", "ifdm_scale_integral", "ifdm_constant", "ifdm_abi_version",
+    "ifdm_set_profiling", "ifdm_get_profile", "ifdm_stream_copy", "ifdm_timestep_criteria", "ifdm_scale_integral", "ifdm_constant", "ifdm_abi_version",
 ]
 
 
@@ -78,6 +78,7 @@ def lib():
     L.ifdm_time_passes.argtypes = [vp, ctypes.POINTER(_Dt), ip, ctypes.POINTER(ctypes.c_float)]
     L.ifdm_fft3d.argtypes = [vp, ip]
     L.ifdm_set_profiling.argtypes = [vp, ip]
+    L.ifdm_stream_copy.argtypes = [vp, ctypes.c_size_t, ip, dp]
     L.ifdm_get_profile.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_long)]
     L.ifdm_timestep_criteria.argtypes = [ctypes.POINTER(_Config), ctypes.c_double, ctypes.POINTER(_Stats),
                                          ctypes.POINTER(_Dt), dp, ctypes.POINTER(ctypes.c_int)]
@@ -236,6 +237,12 @@ class Solver:
         a = ctypes.c_double()
         self._chk(lib().ifdm_simulate(self._h, a0, nsteps, ctypes.byref(a), sched), "ifdm_simulate")
         return a.value, [(d.dKE, d.dPE, d.dQI) for d in sched[:nsteps]]
+
+    def stream_copy(self, nbytes: int, iters: int = 10) -> float:
+        """GB/s of a plain device copy of nbytes (read + write counted)."""
+        g = ctypes.c_double()
+        self._chk(lib().ifdm_stream_copy(self._h, nbytes, iters, ctypes.byref(g)), "ifdm_stream_copy")
+        return g.value
 
     def set_profiling(self, on: bool):
         self._chk(lib().ifdm_set_profiling(self._h, int(on)), "ifdm_set_profiling")

@@ -39,6 +39,11 @@ struct ifdm_ctx {
     double ke_coef = 0;
     double green = 0;
     double fftnorm = 0;
+    // mixed precision: psi is stored as 2^-psi_scale psi (complex64)
+    bool mixed = false;
+    int psi_scale = 0;
+    double psc2 = 1.0;         // 2^(2 psi_scale)
+    double rho_mean = 0.0;     // m_s <|psi|^2>, FP64, subtracted in the mixed-mode source
     // optional live per-pass timing (ifdm_set_profiling)
     bool prof = false;
     hipEvent_t ev[IFDM_NPASS + 1] = {};
@@ -69,9 +74,9 @@ struct Ops {
     static ZArgs zargs(ifdm_ctx *c, double aphi, double arho) {
         ZArgs z;
         z.aphi = aphi;
-        z.arho = arho;
-        z.ms = c->ms;
-        z.rhoc = c->cfg.rhoc;
+        z.arho = arho * c->psc2;
+        z.ms = c->ms * c->psc2;
+        z.rhoc = c->mixed ? c->rho_mean : c->cfg.rhoc;
         z.ref_range = c->cfg.ref_maxphi_range;
         return z;
     }
@@ -175,7 +180,7 @@ struct Ops {
         long nb = (c->n3 + 255) / 256;
         if (nb > 65536) nb = 65536;
         hipLaunchKernelGGL((k_kick<T>), dim3((unsigned)nb), dim3(256), 0, c->stream, (cx<T> *)c->psi,
-                           (const T *)c->phi, c->n3, aphi, arho);
+                           (const T *)c->phi, c->n3, aphi, arho * c->psc2);
         LAUNCHCHK(c);
         return 0;
     }
@@ -197,6 +202,19 @@ Dispatch make_dispatch() {
 }
 
 bool get_dispatch(int n, int precision, Dispatch *d) {
+    if (precision == IFDM_MIXED) {
+        switch (n) {
+            case 8: *d = make_dispatch<float, 8>(); return true;
+            case 16: *d = make_dispatch<float, 16>(); return true;
+            case 32: *d = make_dispatch<float, 32>(); return true;
+            case 64: *d = make_dispatch<float, 64>(); return true;
+            case 128: *d = make_dispatch<float, 128>(); return true;
+            case 256: *d = make_dispatch<float, 256>(); return true;
+            case 512: *d = make_dispatch<float, 512>(); return true;
+            case 1024: *d = make_dispatch<float, 1024>(); return true;
+            default: return false;
+        }
+    }
     if (precision != IFDM_FP64) return false;
     switch (n) {
         case 8: *d = make_dispatch<double, 8>(); return true;
@@ -227,10 +245,12 @@ int accumulate_profile(ifdm_ctx *c) {
 int read_stats(ifdm_ctx *c, ifdm_stats *st) {
     HIPCHK(c, hipMemcpyAsync(c->red_host, c->red, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double maxrho = c->red_host[1] * c->psc2, norm = c->red_host[2] * c->psc2;
+    if (c->mixed) c->rho_mean = c->ms * norm / (double)c->n3;
     if (st) {
         st->max_phi = c->red_host[0];
-        st->max_rho = c->red_host[1];
-        st->norm = c->red_host[2];
+        st->max_rho = maxrho;
+        st->norm = norm;
     }
     return 0;
 }
@@ -270,7 +290,10 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     if (!cfg || !out) { g_create_err = "null argument"; return IFDM_E_ARG; }
     *out = nullptr;
     if (!pow2_in_range(cfg->n)) { g_create_err = "n must be a power of two in [8, 1024]"; return IFDM_E_ARG; }
-    if (cfg->precision != IFDM_FP64) { g_create_err = "precision mode not available"; return IFDM_E_ARG; }
+    if (cfg->precision != IFDM_FP64 && cfg->precision != IFDM_MIXED) {
+        g_create_err = "precision must be IFDM_FP64 or IFDM_MIXED";
+        return IFDM_E_ARG;
+    }
     if (!(cfg->xmax > 0) || !(cfg->mfdm > 0)) { g_create_err = "xmax and mfdm must be positive"; return IFDM_E_ARG; }
     Dispatch d;
     if (!get_dispatch(cfg->n, cfg->precision, &d)) { g_create_err = "unsupported size"; return IFDM_E_ARG; }
@@ -280,6 +303,7 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     c->n = cfg->n;
     c->n3 = (long)cfg->n * cfg->n * cfg->n;
     c->hs = hs_of(cfg->n);
+    c->mixed = cfg->precision == IFDM_MIXED;
     const int n = c->n;
     auto fail = [&](int code, const std::string &m) {
         g_create_err = m;
@@ -290,7 +314,8 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     if (e != hipSuccess) return fail(IFDM_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
         return fail(IFDM_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-    const size_t rs = sizeof(double), cs = 2 * sizeof(double);
+    // element sizes of the stored fields (reductions stay FP64)
+    const size_t rs = c->mixed ? sizeof(float) : sizeof(double), cs = 2 * rs, ds = sizeof(double);
     // z-pass block count: lines per block = min(256 / (n / 8), n * n)
     const int per = std::min(256 / (n / 8), n * n);
     c->nz_blocks = (n * n + per - 1) / per;
@@ -299,27 +324,31 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
         {&c->phi, rs * (size_t)c->n3},
         {&c->shat, cs * (size_t)n * n * c->hs},
         {&c->tw, cs * (size_t)n},
-        {(void **)&c->k2, rs * (size_t)n},
-        {(void **)&c->part_rho, 2 * rs * (size_t)c->nz_blocks},
-        {(void **)&c->part_phi, rs * (size_t)c->nz_blocks},
-        {(void **)&c->red, 3 * rs},
+        {(void **)&c->k2, ds * (size_t)n},
+        {(void **)&c->part_rho, 2 * ds * (size_t)c->nz_blocks},
+        {(void **)&c->part_phi, ds * (size_t)c->nz_blocks},
+        {(void **)&c->red, 3 * ds},
     };
     for (auto &a : allocs) {
         if ((e = hipMalloc(a.p, a.bytes)) != hipSuccess)
             return fail(IFDM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     }
-    if ((e = hipHostMalloc((void **)&c->red_host, 3 * rs)) != hipSuccess)
+    if ((e = hipHostMalloc((void **)&c->red_host, 3 * ds)) != hipSuccess)
         return fail(IFDM_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
     // twiddles W_N^k = exp(-2 pi i k / N) evaluated in long double
     std::vector<double> tw(2 * (size_t)n), k2(n);
+    std::vector<float> twf(2 * (size_t)n);
     for (int k = 0; k < n; ++k) {
         const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
         tw[2 * k] = (double)cosl(a);
         tw[2 * k + 1] = (double)-sinl(a);
+        twf[2 * k] = (float)cosl(a);
+        twf[2 * k + 1] = (float)-sinl(a);
         k2[k] = std::pow(kvec(k, n, cfg->xmax), 2);
     }
-    if ((e = hipMemcpy(c->tw, tw.data(), cs * n, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(c->k2, k2.data(), rs * n, hipMemcpyHostToDevice)) != hipSuccess ||
+    const void *twsrc = c->mixed ? (const void *)twf.data() : (const void *)tw.data();
+    if ((e = hipMemcpy(c->tw, twsrc, cs * n, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(c->k2, k2.data(), ds * n, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemset(c->psi, 0, cs * (size_t)c->n3)) != hipSuccess ||
         (e = hipMemset(c->phi, 0, rs * (size_t)c->n3)) != hipSuccess ||
         (e = hipMemset(c->shat, 0, cs * (size_t)n * n * c->hs)) != hipSuccess)
@@ -331,6 +360,13 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     c->ke_coef = 0.5 * hbar / mt;                            // src/struct3D.h:121
     c->green = -4.0 * M_PI * bigG;                           // src/CosmoBEC_MPI_stan.cpp:105
     c->fftnorm = 1. / static_cast<double>(std::pow(n, 3));   // src/struct3D.h:69
+    if (c->mixed) {
+        // |psi| ~ sqrt(rho_c / m_s) ~ 1e50 overflows FP32: store 2^-s psi with
+        // |2^-s psi|^2 ~ 1 at the mean density (an exact rescaling)
+        c->psi_scale = (int)std::lround(0.5 * std::log2(std::max(cfg->rhoc, 1e-300) / c->ms));
+        c->psc2 = std::ldexp(1.0, 2 * c->psi_scale);
+        c->rho_mean = cfg->rhoc;
+    }
     *out = c;
     return IFDM_OK;
 }
@@ -340,32 +376,61 @@ void ifdm_destroy(ifdm_ctx *ctx) { release(ctx); }
 int ifdm_set_psi(ifdm_ctx *c, const double *psi) {
     if (!c || !psi) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
-    HIPCHK(c, hipMemcpyAsync(c->psi, psi, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!c->mixed) {
+        HIPCHK(c, hipMemcpyAsync(c->psi, psi, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return IFDM_OK;
+    }
+    std::vector<float> tmp(2 * (size_t)c->n3);
+    double norm = 0.0;
+    for (size_t i = 0; i < 2 * (size_t)c->n3; ++i) {
+        tmp[i] = (float)std::ldexp(psi[i], -c->psi_scale);
+        norm += psi[i] * psi[i];
+    }
+    c->rho_mean = c->ms * norm / (double)c->n3;
+    HIPCHK(c, hipMemcpy(c->psi, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice));
     return IFDM_OK;
 }
 
 int ifdm_get_psi(ifdm_ctx *c, double *psi) {
     if (!c || !psi) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
-    HIPCHK(c, hipMemcpyAsync(psi, c->psi, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!c->mixed) {
+        HIPCHK(c, hipMemcpy(psi, c->psi, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToHost));
+        return IFDM_OK;
+    }
+    std::vector<float> tmp(2 * (size_t)c->n3);
+    HIPCHK(c, hipMemcpy(tmp.data(), c->psi, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < tmp.size(); ++i) psi[i] = std::ldexp((double)tmp[i], c->psi_scale);
     return IFDM_OK;
 }
 
 int ifdm_get_phi(ifdm_ctx *c, double *phi) {
     if (!c || !phi) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
-    HIPCHK(c, hipMemcpyAsync(phi, c->phi, sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!c->mixed) {
+        HIPCHK(c, hipMemcpy(phi, c->phi, sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToHost));
+        return IFDM_OK;
+    }
+    std::vector<float> tmp((size_t)c->n3);
+    HIPCHK(c, hipMemcpy(tmp.data(), c->phi, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < tmp.size(); ++i) phi[i] = tmp[i];
     return IFDM_OK;
 }
 
 int ifdm_set_phi(ifdm_ctx *c, const double *phi) {
     if (!c || !phi) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
-    HIPCHK(c, hipMemcpyAsync(c->phi, phi, sizeof(double) * (size_t)c->n3, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!c->mixed) {
+        HIPCHK(c, hipMemcpy(c->phi, phi, sizeof(double) * (size_t)c->n3, hipMemcpyHostToDevice));
+        return IFDM_OK;
+    }
+    std::vector<float> tmp((size_t)c->n3);
+    for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)phi[i];
+    HIPCHK(c, hipMemcpy(c->phi, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice));
     return IFDM_OK;
 }
 
@@ -474,6 +539,42 @@ int ifdm_simulate(ifdm_ctx *c, double a0, int nsteps, double *a_out, ifdm_dt *sc
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (a_out) *a_out = a;
     return IFDM_OK;
+}
+
+int ifdm_stream_copy(ifdm_ctx *c, size_t bytes, int iters, double *gbs) {
+    if (!c || !gbs || iters <= 0 || bytes < 16) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const long n = (long)(bytes / 16);
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(c, hipMalloc(&a, n * 16));
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        hipFree(a);
+        c->err = "hipMalloc failed in ifdm_stream_copy";
+        return IFDM_E_NOMEM;
+    }
+    hipEvent_t e0, e1;
+    int rc = IFDM_OK;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipMemsetAsync(a, 0, n * 16, c->stream);
+    const unsigned nb = (unsigned)std::min<long>((n + 255) / 256, 1L << 20);
+    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, c->stream, (const double2 *)a, (double2 *)b, n);  // warm
+    hipEventRecord(e0, c->stream);
+    for (int i = 0; i < iters; ++i)
+        hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, c->stream, (const double2 *)a, (double2 *)b, n);
+    hipEventRecord(e1, c->stream);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess ||
+        hipGetLastError() != hipSuccess) {
+        c->err = "copy kernel failed";
+        rc = IFDM_E_HIP;
+    }
+    *gbs = 2.0 * 16.0 * (double)n * iters / (ms * 1e-3) / 1e9;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    return rc;
 }
 
 int ifdm_set_profiling(ifdm_ctx *c, int on) {

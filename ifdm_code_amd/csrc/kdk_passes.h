@@ -33,7 +33,8 @@ struct ZArgs {
     double aphi;      // kick: phase coefficient of phi
     double arho;      // kick: phase coefficient of |psi|^2
     double ms;        // Poisson source: m_s (times 2^(2s) for a scaled psi)
-    double rhoc;      // Poisson source: subtracted mean density
+    double rhoc;      // Poisson source: subtracted density (rho_c; the FP64
+                      // mean m_s <|psi|^2> in mixed mode)
     int ref_range;    // max|phi| over the reference's one-rank scan range
 };
 
@@ -53,6 +54,23 @@ template <int N> constexpr int z_lines() {
     return (256 / P) < (N * N) ? (256 / P) : (N * N);
 }
 template <typename T> constexpr int s_cols() { return 128 / (int)sizeof(cx<T>); }  // 128-byte rows
+
+// exp(-i th) as the storage precision: FP64 sincos, or (mixed mode) the
+// FP64 argument reduced mod 2 pi in FP64 (two-term 2 pi) and an FP32 sincos.
+template <typename T> __device__ __forceinline__ cx<T> expmi(double th);
+template <> __device__ __forceinline__ cx<double> expmi<double>(double th) {
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    return mkc<double>(cs, -sn);
+}
+template <> __device__ __forceinline__ cx<float> expmi<float>(double th) {
+    const double k = rint(th * 0.15915494309189533577);
+    double r = fma(-k, 6.283185307179586232, th);
+    r = fma(-k, 2.4492935982947064e-16, r);
+    float sn, cs;
+    sincosf((float)r, &sn, &cs);
+    return mkc<float>(cs, -sn);
+}
 
 // block reduction over the whole work-group; sbuf holds >= nthreads doubles
 template <bool MAX>
@@ -120,10 +138,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             for (int e = 0; e < E; ++e) {
                 const double f = valid ? (double)prow[t + e * P] : 0.0;
                 const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
-                const double th = za.aphi * f + za.arho * rr;
-                double sn, cs;
-                sincos(th, &sn, &cs);
-                v[e] = cmul(v[e], mkc<T>((T)cs, (T)-sn));
+                v[e] = cmul(v[e], expmi<T>(za.aphi * f + za.arho * rr));
             }
         }
         if constexpr (MODE == Z_FWD || MODE == Z_KICK_FWD) {
@@ -199,10 +214,8 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
             const int i = t + e * P;
             const double k2 = sa.k2[i] + k2o + k2k;
             if constexpr (MODE == S_DRIFT) {
-                const double th = -(sa.ke_coef * k2) * sa.dKE;
-                double sn, cs;
-                sincos(th, &sn, &cs);
-                v[e] = cmul(v[e], mkc<T>((T)(cs * sa.fftnorm), (T)(sn * sa.fftnorm)));
+                const cx<T> ph = expmi<T>((sa.ke_coef * k2) * sa.dKE);
+                v[e] = cmul(v[e], cscale(ph, (T)sa.fftnorm));
             } else {
                 const double gf = (k2 == 0.0) ? 0.0 : sa.green / k2 * sa.fftnorm;
                 v[e] = cscale(v[e], (T)gf);
@@ -223,11 +236,14 @@ __global__ __launch_bounds__(256) void k_kick(cx<T> *__restrict__ psi, const T *
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
         cx<T> z = psi[i];
         const double rr = (double)z.x * z.x + (double)z.y * z.y;
-        const double th = aphi * (double)phi[i] + arho * rr;
-        double sn, cs;
-        sincos(th, &sn, &cs);
-        psi[i] = cmul(z, mkc<T>((T)cs, (T)-sn));
+        psi[i] = cmul(z, expmi<T>(aphi * (double)phi[i] + arho * rr));
     }
+}
+
+// plain 16-byte-vector copy: the achievable streaming bandwidth reference
+__global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 // final reduction of the per-block partials: out = {max|phi|, max|psi|^2, sum|psi|^2}

@@ -130,6 +130,11 @@ int ifdm_time_passes(ifdm_ctx *ctx, const ifdm_dt *dt, int iters, float *ms_out)
 int ifdm_set_profiling(ifdm_ctx *ctx, int on);
 int ifdm_get_profile(const ifdm_ctx *ctx, double *ms_sum, long *count);
 
+/* Streaming-bandwidth reference: copies `bytes` between two scratch device
+ * buffers `iters` times with a 16-byte-vector kernel and reports the
+ * achieved (read + write) GB/s.  Allocates and frees the two buffers. */
+int ifdm_stream_copy(ifdm_ctx *ctx, size_t bytes, int iters, double *gbs);
+
 /* The drift alone (forward 3-D FFT, exp(-i KE_phase dKE)/N^3, inverse),
  * src/CosmoBEC_MPI_stan.cpp:136-144. */
 int ifdm_drift(ifdm_ctx *ctx, double dKE);

@@ -299,3 +299,154 @@ void oracle_set_threads(int t) {
     (void)t;
 #endif
 }
+
+/* ------------------------------------------------------------- IC path */
+
+/* CIC_scheme, src/gadget_IC.h:45-156, at one rank (my_rank = 0,
+ * local_n0 = N).  rho: N^3 doubles (Re of opr.wfc), vx/vy/vz: N^3 floats
+ * (vxg, vyg, vzg are std::vector<float>, :583).  Positions in kpc/h,
+ * box in kpc/h, pmass in Msun/h (header mass[1] * 1e10, :317).  xg[i] =
+ * i*Lg + Lg/2 (:333). */
+static void cic_one(int Ng, double box, double pmass, double Lg, const double *xg,
+                    float xp, float yp, float zp, float vxp, float vyp, float vzp,
+                    double *rho, float *vx, float *vy, float *vz) {
+    int gi = (int)floor((xp - 0.5 * Lg) / Lg);
+    int gj = (int)floor((yp - 0.5 * Lg) / Lg);
+    int gk = (int)floor((zp - 0.5 * Lg) / Lg);
+    if (gi == -1) { gi = Ng - 1; xp += box; }
+    if (gj < 0) { gj = Ng - 1; yp += box; }
+    if (gk < 0) { gk = Ng - 1; zp += box; }
+    double dx = fabs((double)xp - xg[gi]) / Lg;
+    double dy = fabs((double)yp - xg[gj]) / Lg;
+    double dz = fabs((double)zp - xg[gk]) / Lg;
+    double tx = 1. - dx, ty = 1. - dy, tz = 1. - dz;
+    int gii = gi + 1, gjj = gj + 1, gkk = gk + 1;
+    if (gii == Ng) gii = 0;
+    if (gjj == Ng) gjj = 0;
+    if (gkk == Ng) gkk = 0;
+    double L3 = pow(Lg, 3);
+    /* index k + Ng*(j + i*Ng) */
+#define IDX(i, j, k) ((long)(k) + (long)Ng * ((long)(j) + (long)(i) * Ng))
+    if (gi >= 0 && gi < Ng) {
+        rho[IDX(gi, gj, gk)] += tx * ty * tz * pmass / L3;
+        rho[IDX(gi, gjj, gk)] += tx * dy * tz * pmass / L3;
+        rho[IDX(gi, gj, gkk)] += tx * ty * dz * pmass / L3;
+        rho[IDX(gi, gjj, gkk)] += tx * dy * dz * pmass / L3;
+        float *vs[3] = {vx, vy, vz};
+        float vp[3] = {vxp, vyp, vzp};
+        for (int c = 0; c < 3; ++c) {
+            vs[c][IDX(gi, gj, gk)] += tx * ty * tz * vp[c] / L3;
+            vs[c][IDX(gi, gjj, gk)] += tx * dy * tz * vp[c] / L3;
+            vs[c][IDX(gi, gj, gkk)] += tx * ty * dz * vp[c] / L3;
+            vs[c][IDX(gi, gjj, gkk)] += tx * dy * dz * vp[c] / L3;
+        }
+    }
+    if (gii >= 0 && gii < Ng) {
+        rho[IDX(gii, gjj, gk)] += dx * dy * tz * pmass / L3;
+        rho[IDX(gii, gj, gk)] += dx * ty * tz * pmass / L3;
+        rho[IDX(gii, gj, gkk)] += dx * ty * dz * pmass / L3;
+        rho[IDX(gii, gjj, gkk)] += dx * dy * dz * pmass / L3;
+        float *vs[3] = {vx, vy, vz};
+        float vp[3] = {vxp, vyp, vzp};
+        for (int c = 0; c < 3; ++c) {
+            vs[c][IDX(gii, gjj, gk)] += dx * dy * tz * vp[c] / L3;
+            vs[c][IDX(gii, gj, gk)] += dx * ty * tz * vp[c] / L3;
+            vs[c][IDX(gii, gj, gkk)] += dx * ty * dz * vp[c] / L3;
+            vs[c][IDX(gii, gjj, gkk)] += dx * dy * dz * vp[c] / L3;
+        }
+    }
+#undef IDX
+}
+
+/* Deposit one IC chunk of np particles (pos, vel: 3*np floats) the way
+ * read_gadget does at one rank (src/gadget_IC.h:610-642).  ref_p1 != 0
+ * also deposits the rank's "left ghost" copies: the trailing particles of
+ * the x-sorted chunk with x > xg[N-1] (Pre_DistributeParticles_withGhost,
+ * :445-448), which at one rank are then deposited twice.  The extra particle
+ * the reference reads past the end of the chunk (:431-432) is not
+ * reproduced. */
+void oracle_cic_chunk(int Ng, double box, double pmass, const float *pos, const float *vel, long np,
+                      int ref_p1, double *rho, float *vx, float *vy, float *vz) {
+    double Lg = box / Ng;
+    double *xg = (double *)malloc(sizeof(double) * Ng);
+    for (int i = 0; i < Ng; ++i) xg[i] = i * Lg + 0.5 * Lg;
+    if (ref_p1) {
+        long ng = 0;
+        for (long pn = np - 1; pn > 0; --pn) {
+            if (xg[Ng - 1] < pos[3 * pn]) ++ng;
+            else break;
+        }
+        for (long q = np - ng; q < np; ++q)
+            cic_one(Ng, box, pmass, Lg, xg, pos[3 * q], pos[3 * q + 1], pos[3 * q + 2],
+                    vel[3 * q], vel[3 * q + 1], vel[3 * q + 2], rho, vx, vy, vz);
+    }
+    for (long q = 0; q < np; ++q)
+        cic_one(Ng, box, pmass, Lg, xg, pos[3 * q], pos[3 * q + 1], pos[3 * q + 2],
+                vel[3 * q], vel[3 * q + 1], vel[3 * q + 2], rho, vx, vy, vz);
+    free(xg);
+}
+
+/* After all chunks: mass-weighted velocity (src/gadget_IC.h:645-651), the
+ * velocity-divergence phase (initial_velocity_Poisson_Solver, :157-247) and
+ * the wave function (:659-670).  psi (out) N^3 complex; theta (out, may be
+ * NULL) N^3 complex, the phase field opr.phik.  a: scale factor of the IC
+ * header; dx: cell size in Mpc (par.dx). */
+void oracle_ic_assemble(const oracle_params *p, double pmass, double a, double dx, double h0,
+                        double m2Mpc_, double s2Myr_, double *rho, float *vx, float *vy, float *vz,
+                        double *psi, double *theta_out) {
+    int N = p->n;
+    long n3 = (long)N * N * N;
+    for (long i = 0; i < n3; ++i) {
+        if (rho[i] != 0) {
+            vx[i] *= pmass / rho[i];
+            vy[i] *= pmass / rho[i];
+            vz[i] *= pmass / rho[i];
+        }
+    }
+    cd *th = (cd *)malloc(sizeof(cd) * (size_t)n3);
+    double unit = 1e3 / m2Mpc_ * s2Myr_;                         /* :171 */
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < N; ++k) {
+                int ip = (i + 1) % N, im = (i + N - 1) % N;     /* ghost planes at one rank */
+                int jp = (j + 1) % N, jm = (j + N - 1) % N;
+                int kp = (k + 1) % N, km = (k + N - 1) % N;
+                float dvx = vx[k + (long)N * (j + (long)ip * N)] - vx[k + (long)N * (j + (long)im * N)];
+                float dvy = vy[k + (long)N * (jp + (long)i * N)] - vy[k + (long)N * (jm + (long)i * N)];
+                float dvz = vz[kp + (long)N * (j + (long)i * N)] - vz[km + (long)N * (j + (long)i * N)];
+                long o = k + (long)N * (j + (long)i * N);
+                th[o].re = 0.5 * (dvx + dvy + dvz) * unit / dx;  /* :216 */
+                th[o].im = 0.0;
+            }
+    fft_axis(th, N, N, N, 2, -1);
+    fft_axis(th, N, N, N, 1, -1);
+    fft_axis(th, N, N, N, 0, -1);
+    double a32 = pow(a, 3. / 2);
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < N; ++k) {
+                double kx = kvec(i, N, p->xmax), ky = kvec(j, N, p->xmax), kz = kvec(k, N, p->xmax);
+                double k2 = pow(kx, 2) + pow(ky, 2) + pow(kz, 2);
+                long o = k + (long)N * (j + (long)i * N);
+                if (k2 == 0) { th[o].re = 0.0; th[o].im = 0.0; }
+                else { th[o].re = -a32 * th[o].re / k2; th[o].im = -a32 * th[o].im / k2; }   /* :233 */
+            }
+    fft_axis(th, N, N, N, 2, +1);
+    fft_axis(th, N, N, N, 1, +1);
+    fft_axis(th, N, N, N, 0, +1);
+    double nn = (double)((long)N * N * N);
+    for (long i = 0; i < n3; ++i) { th[i].re /= nn; th[i].im /= nn; }   /* :241-244 */
+    double uc = pow(h0, 2) * pow(1e3, 3);                               /* :659 */
+    double cph = p->mfdm / pow(p->c, 2) / p->hbar;
+    cd *ps = (cd *)psi;
+    for (long i = 0; i < n3; ++i) {
+        /* 1.225 sqrt(rho u / m_s) exp(c theta i), theta complex (:666) */
+        double amp = 1.225 * sqrt(rho[i] * uc / p->mfdm_solar);
+        double er = -cph * th[i].im, ei = cph * th[i].re;   /* exp(i c (re + i im)) = exp(-c im) e^{i c re} */
+        double m = exp(er);
+        ps[i].re = amp * m * cos(ei);
+        ps[i].im = amp * m * sin(ei);
+    }
+    if (theta_out) memcpy(theta_out, th, sizeof(cd) * (size_t)n3);
+    free(th);
+}

@@ -201,3 +201,57 @@ def test_time_reversal():
         s.split_op(*dt)
         back = np.conj(s.get_psi())
     assert rel_l2(back, psi) < 1e-12
+
+
+# ---------------------------------------------------------------- mixed mode
+def _cfg_mixed(F, n, g_in=0.0):
+    return F.Config.from_cli(2.5e-24, g_in, 5.0, n, precision=F.MIXED)
+
+
+@pytest.mark.parametrize("n", [16, 64, 128])
+def test_mixed_fft_and_poisson(n):
+    """complex64 storage / FP32 butterflies: single transforms to ~1e-6."""
+    F, O = _mods()
+    _say(f"[mixed fft/poisson] n={n}")
+    cfg = _cfg_mixed(F, n)
+    psi = make_psi(n, 31, cfg.mfdm_solar)
+    st_o = O.State(_rp(O, n), psi)
+    st_o.poisson()
+    with F.Solver(cfg) as s:
+        s.set_psi(psi)
+        back = s.get_psi()
+        assert rel_l2(back, psi) < 1e-7          # power-of-two scaling is exact; FP32 rounding only
+        stats = s.poisson()
+        phi = s.get_phi()
+        m = st_o.maxima()
+        assert rel_l2(phi, st_o.phi) < 2e-5
+        assert stats["norm"] == pytest.approx(m["norm"], rel=1e-6)
+        assert stats["max_rho"] == pytest.approx(m["max_rho"], rel=1e-6)
+        s.fft3d(-1)
+        f = s.get_psi()
+    assert rel_l2(f, O.fft3d_c2c(psi, -1)) < 2e-6
+
+
+@pytest.mark.parametrize("n,steps", [(64, 20)])
+def test_mixed_run_density_error(n, steps):
+    """Mixed-mode replay of an FP64 schedule: density relative L2 vs the
+    FP64 oracle stays well under 1e-4 and the norm is conserved to FP32
+    round-off."""
+    F, O = _mods()
+    _say(f"[mixed run] n={n} steps={steps}")
+    cfg = _cfg_mixed(F, n, g_in=1e-56)
+    rp = _rp(O, n, g_in=1e-56)
+    psi = make_psi(n, 77, cfg.mfdm_solar)
+    st_o, used = O.simulate(rp, psi, 1.0 / 51, steps)
+    sched = [u[:3] for u in used]
+    with F.Solver(cfg) as s:
+        s.set_psi(psi)
+        s.poisson()
+        s.run(sched)
+        out = s.get_psi()
+        st = s.poisson()
+    rho, rho_o = np.abs(out) ** 2, np.abs(st_o.psi) ** 2
+    err = rel_l2(rho, rho_o)
+    _say(f"   mixed density rel L2 = {err:.3e}")
+    assert err < 1e-4
+    assert abs(st["norm"] / float(np.sum(np.abs(psi) ** 2)) - 1) < 1e-5
