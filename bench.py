@@ -95,20 +95,27 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-def timed_adaptive(F, cfg, psi, warmup: int, steps: int):
-    """warm-up + timed adaptive steps (ifdm_simulate) with live per-pass
-    events; returns (steps/s, avg ms per pass, schedule, final psi, solver a)."""
+def timed_adaptive(F, cfg, psi, warmup: int, steps: int, slab: int, prof_steps: int, keep_state: bool):
+    """warm-up, then `steps` timed adaptive steps (ifdm_simulate, no events),
+    then `prof_steps` more with live per-launch HIP events for the per-pass
+    times.  Returns (steps/s, seconds, avg ms per pass, schedule of the
+    warm-up + timed steps, psi and phi after them (or None), a)."""
     s = F.Solver(cfg)
     s.set_psi(psi)
+    s.set_option("slab_planes", slab)
     a, sched_w = s.simulate_native(1.0 / 51, warmup)
-    s.set_profiling(True)
     t0 = time.perf_counter()
     a, sched_t = s.simulate_native(a, steps)
     el = time.perf_counter() - t0
+    psi_out = s.get_psi() if keep_state else None
+    phi_out = s.get_phi() if keep_state else None
+    s.set_profiling(True)
+    s.simulate_native(a, prof_steps)
     ms_sum, cnt = s.get_profile()
     s.set_profiling(False)
+    s.close()
     avg = [m / max(cnt, 1) for m in ms_sum]
-    return s, steps / el, el, avg, sched_w + sched_t, a
+    return steps / el, el, avg, sched_w + sched_t, psi_out, phi_out, a
 
 
 def counter_traffic(n: int, precision: str) -> dict:
@@ -145,6 +152,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--slab", type=int, default=-1, help="x-planes per slab chain (-1: library default)")
+    ap.add_argument("--prof-steps", type=int, default=5)
+    ap.add_argument("--sweep-slab", type=str, default="", help="comma list of slab sizes to time (FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -176,11 +186,15 @@ def main():
     cfg = F.Config.from_cli(2.5e-24, g_in, xmax_in, n)
     log(f"[bench] g = {g_in:.4e} eV cm^3")
 
-    s, sps, el, avg, sched, a = timed_adaptive(F, cfg, psi, args.warmup, args.steps)
+    slab = args.slab if args.slab >= 0 else F.default_slab(n)
+    sweep = {}
+    for sv in [int(x) for x in args.sweep_slab.split(",") if x.strip()]:
+        r = timed_adaptive(F, cfg, psi, args.warmup, args.steps, sv, 0, False)
+        sweep[sv] = round(r[0], 3)
+        log(f"[bench] slab {sv}: {r[0]:.2f} steps/s")
+    sps, el, avg, sched, psi64, phi64, a = timed_adaptive(F, cfg, psi, args.warmup, args.steps, slab,
+                                                          args.prof_steps, not args.no_mixed)
     log(f"[bench] FP64: {args.steps} steps in {el:.3f} s ({sps:.2f} steps/s), a = {a:.5f}")
-    psi64 = s.get_psi() if not args.no_mixed else None
-    phi64 = s.get_phi() if not args.no_mixed else None
-    s.close()
     pb = pass_bytes(n)
     b_alg = sum(pb)
     step_gbs = b_alg * sps / 1e9
@@ -191,7 +205,7 @@ def main():
         "data": "synthetic: seeded Gaussian-random-field psi (delta rms 0.05, phase rms 1 rad, P(k)~k^-2.5)",
         "config": {"workload": f"KDK step at {n}^3, box {2 * xmax_in:g} Mpc/h, z=50, mfdm=2.5e-24 eV, "
                                f"g={g_in:.3e} eV cm^3 (1% of gravity), adaptive dt, FP64",
-                   "n": n, "precision": "fp64"},
+                   "n": n, "precision": "fp64", "slab_planes": slab},
         "roofline": roofline(n, avg, 16, 8, "fp64"),
         "step_hbm": {"alg_bytes_per_step": b_alg, "achieved_GBs": round(step_gbs, 1),
                      "frac_of_8TBs": round(step_gbs / HBM_PEAK_GBS, 4),
@@ -199,14 +213,17 @@ def main():
                      "stream_copy_GBs": round(copy_gbs, 1)},
         "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avg)},
     }
+    if sweep:
+        out["slab_sweep_steps_per_s"] = sweep
 
     if not args.no_mixed:
         cfgm = F.Config.from_cli(2.5e-24, g_in, xmax_in, n, precision=F.MIXED)
-        sm, spsm, elm, avgm, _schm, am = timed_adaptive(F, cfgm, psi, args.warmup, args.steps)
-        sm.close()
+        spsm, elm, avgm, _schm, _p, _f, am = timed_adaptive(F, cfgm, psi, args.warmup, args.steps, slab,
+                                                            args.prof_steps, False)
         log(f"[bench] mixed: {args.steps} steps in {elm:.3f} s ({spsm:.2f} steps/s)")
         # error: the mixed path replays the FP64 run's schedule from the same IC
         with F.Solver(cfgm) as sr:
+            sr.set_option("slab_planes", slab)
             sr.set_psi(psi)
             sr.poisson()
             sr.run([tuple(x) for x in sched])

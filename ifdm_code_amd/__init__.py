@@ -44,11 +44,44 @@ class _Dt(ctypes.Structure):
     _fields_ = [("dKE", ctypes.c_double), ("dPE", ctypes.c_double), ("dQI", ctypes.c_double)]
 
 
+class _Trig(ctypes.Structure):
+    _fields_ = [("write_num", ctypes.c_int), ("sten_counter", ctypes.c_int), ("write_sten_counter", ctypes.c_int)]
+
+
+class OutputTriggers:
+    """The snapshot trigger chain of the reference's time loop
+    (src/CosmoBEC_MPI_stan.cpp:238-284; variant 1 = src/CosmoBEC_MPI.cpp)."""
+
+    def __init__(self, variant: int = 0, a_triggers: bool = True):
+        self.variant = variant
+        self.a_triggers = a_triggers
+        self.st = _Trig(0, 0, 2)
+
+    def __call__(self, a: float, tn: int, twrite: int, t_stop: int):
+        """(snapshot number or None, stop flag) for this step."""
+        t, stop = ctypes.c_int(), ctypes.c_int()
+        r = lib().ifdm_output_trigger(self.variant, int(self.a_triggers), a, tn, twrite, t_stop,
+                                      ctypes.byref(self.st), ctypes.byref(t), ctypes.byref(stop))
+        if r < 0:
+            raise IfdmError("ifdm_output_trigger: bad arguments")
+        return (t.value if r == 1 else None), bool(stop.value)
+
+
+class _ICInfo(ctypes.Structure):
+    _fields_ = [("box", ctypes.c_double), ("pmass", ctypes.c_double), ("a", ctypes.c_double),
+                ("rhoc", ctypes.c_double), ("npart", ctypes.c_longlong), ("skipped", ctypes.c_longlong)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 EXPORTS = [
+    "ifdm_read_gadget", "ifdm_ic_begin", "ifdm_ic_deposit", "ifdm_ic_finish", "ifdm_set_rhoc",
+    "ifdm_write_snapshot", "ifdm_read_snapshot",
     "ifdm_create", "ifdm_destroy", "ifdm_last_error", "ifdm_create_error", "ifdm_set_psi", "ifdm_get_psi",
-    "ifdm_get_phi", "ifdm_set_phi", "ifdm_poisson", "ifdm_kick", "ifdm_kdk_step", "ifdm_run", "ifdm_drift",
-    "ifdm_fft3d", "ifdm_simulate", "ifdm_time_passes",
-    "ifdm_set_profiling", "ifdm_get_profile", "ifdm_stream_copy", "ifdm_timestep_criteria", "ifdm_scale_integral", "ifdm_constant", "ifdm_abi_version",
+    "ifdm_get_phi", "ifdm_set_phi", "ifdm_poisson", "ifdm_kick", "ifdm_kdk_step", "ifdm_step", "ifdm_flush", "ifdm_run", "ifdm_drift",
+    "ifdm_fft3d", "ifdm_output_trigger", "ifdm_simulate", "ifdm_time_passes",
+    "ifdm_set_profiling", "ifdm_get_profile", "ifdm_stream_copy", "ifdm_set_option", "ifdm_timestep_criteria", "ifdm_scale_integral", "ifdm_constant", "ifdm_abi_version",
 ]
 
 
@@ -73,15 +106,29 @@ def lib():
     L.ifdm_kick.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.ifdm_kdk_step.argtypes = [vp, ctypes.POINTER(_Dt), ctypes.POINTER(_Stats)]
     L.ifdm_run.argtypes = [vp, ctypes.POINTER(_Dt), ip, ctypes.POINTER(_Stats)]
+    L.ifdm_step.argtypes = [vp, ctypes.POINTER(_Dt), ctypes.POINTER(_Stats)]
+    L.ifdm_flush.argtypes = [vp]
     L.ifdm_drift.argtypes = [vp, ctypes.c_double]
     L.ifdm_simulate.argtypes = [vp, ctypes.c_double, ip, dp, ctypes.POINTER(_Dt)]
     L.ifdm_time_passes.argtypes = [vp, ctypes.POINTER(_Dt), ip, ctypes.POINTER(ctypes.c_float)]
     L.ifdm_fft3d.argtypes = [vp, ip]
+    fp = ctypes.POINTER(ctypes.c_float)
+    ll = ctypes.c_longlong
+    L.ifdm_read_gadget.argtypes = [vp, ctypes.c_char_p, ip, ip, ctypes.c_char_p, ctypes.POINTER(_ICInfo)]
+    L.ifdm_ic_begin.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+    L.ifdm_ic_deposit.argtypes = [vp, fp, fp, ll, ip]
+    L.ifdm_ic_finish.argtypes = [vp, ctypes.c_double, dp, fp, ctypes.POINTER(ll)]
+    L.ifdm_set_rhoc.argtypes = [vp, ctypes.c_double]
+    L.ifdm_write_snapshot.argtypes = [vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    L.ifdm_read_snapshot.argtypes = [vp, ctypes.c_char_p, dp, dp, dp]
     L.ifdm_set_profiling.argtypes = [vp, ip]
     L.ifdm_stream_copy.argtypes = [vp, ctypes.c_size_t, ip, dp]
+    L.ifdm_set_option.argtypes = [vp, ctypes.c_char_p, ctypes.c_long]
     L.ifdm_get_profile.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_long)]
     L.ifdm_timestep_criteria.argtypes = [ctypes.POINTER(_Config), ctypes.c_double, ctypes.POINTER(_Stats),
                                          ctypes.POINTER(_Dt), dp, ctypes.POINTER(ctypes.c_int)]
+    L.ifdm_output_trigger.argtypes = [ip, ip, ctypes.c_double, ip, ip, ip, ctypes.POINTER(_Trig),
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.ifdm_scale_integral.argtypes = [ctypes.c_double, ctypes.c_double, ip]
     L.ifdm_scale_integral.restype = ctypes.c_double
     L.ifdm_constant.argtypes = [ctypes.c_char_p]
@@ -138,6 +185,11 @@ def timestep_criteria(cfg: Config, a: float, stats: dict):
     if rc:
         raise IfdmError(f"ifdm_timestep_criteria failed ({rc})")
     return (dt.dKE, dt.dPE, dt.dQI), an.value, which.value
+
+
+def default_slab(n: int) -> int:
+    """x-planes per slab chain used by the bench unless told otherwise."""
+    return 0
 
 
 def scale_integral(a0: float, a1: float, power: int) -> float:
@@ -201,6 +253,43 @@ class Solver:
             raise ValueError("phi size mismatch")
         self._chk(lib().ifdm_set_phi(self._h, _dp(a)), "ifdm_set_phi")
 
+    # ---- initial conditions and snapshots
+    def read_gadget(self, icpath: str, nchunks: int = 4, ref_ghosts: bool = False, voutput: str | None = None):
+        info = _ICInfo()
+        self._chk(lib().ifdm_read_gadget(self._h, icpath.encode(), nchunks, int(ref_ghosts),
+                                         voutput.encode() if voutput else None, ctypes.byref(info)),
+                  "ifdm_read_gadget")
+        return info.as_dict()
+
+    def ic_from_particles(self, chunks, box: float, pmass: float, a: float, ref_ghosts: bool = False,
+                          want_theta: bool = False):
+        """chunks: iterable of (pos, vel) float32 arrays of shape (np, 3)."""
+        self._chk(lib().ifdm_ic_begin(self._h, box, pmass), "ifdm_ic_begin")
+        fp = ctypes.POINTER(ctypes.c_float)
+        for pos, vel in chunks:
+            p = np.ascontiguousarray(pos, dtype=np.float32)
+            v = np.ascontiguousarray(vel, dtype=np.float32)
+            self._chk(lib().ifdm_ic_deposit(self._h, p.ctypes.data_as(fp), v.ctypes.data_as(fp), p.shape[0],
+                                            int(ref_ghosts)), "ifdm_ic_deposit")
+        n = self.cfg.n
+        theta = np.empty((n, n, n), np.complex128) if want_theta else None
+        sk = ctypes.c_longlong()
+        self._chk(lib().ifdm_ic_finish(self._h, a, _dp(theta.view(np.float64)) if want_theta else None, None,
+                                       ctypes.byref(sk)), "ifdm_ic_finish")
+        return theta, sk.value
+
+    def set_rhoc(self, rhoc: float):
+        self._chk(lib().ifdm_set_rhoc(self._h, rhoc), "ifdm_set_rhoc")
+
+    def write_snapshot(self, path: str, a: float, mfdm_ev: float, g_ev_cm3: float):
+        self._chk(lib().ifdm_write_snapshot(self._h, path.encode(), a, mfdm_ev, g_ev_cm3), "ifdm_write_snapshot")
+
+    def read_snapshot(self, path: str):
+        a, m, g = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._chk(lib().ifdm_read_snapshot(self._h, path.encode(), ctypes.byref(a), ctypes.byref(m),
+                                           ctypes.byref(g)), "ifdm_read_snapshot")
+        return a.value, m.value, g.value
+
     # ---- the hot path
     def poisson(self) -> dict:
         st = _Stats()
@@ -221,6 +310,15 @@ class Solver:
         dt = _Dt(dKE, dPE, dQI)
         self._chk(lib().ifdm_kdk_step(self._h, ctypes.byref(dt), ctypes.byref(st)), "ifdm_kdk_step")
         return st.as_dict()
+
+    def step(self, dKE: float, dPE: float, dQI: float) -> dict:
+        """One step with the second half-kick deferred (ifdm_step)."""
+        st = _Stats()
+        self._chk(lib().ifdm_step(self._h, ctypes.byref(_Dt(dKE, dPE, dQI)), ctypes.byref(st)), "ifdm_step")
+        return st.as_dict()
+
+    def flush(self):
+        self._chk(lib().ifdm_flush(self._h), "ifdm_flush")
 
     def run(self, schedule, stats: bool = False):
         """Replay a schedule [(dKE, dPE, dQI), ...] with merged half-kicks."""
@@ -243,6 +341,9 @@ class Solver:
         g = ctypes.c_double()
         self._chk(lib().ifdm_stream_copy(self._h, nbytes, iters, ctypes.byref(g)), "ifdm_stream_copy")
         return g.value
+
+    def set_option(self, name: str, value: int):
+        self._chk(lib().ifdm_set_option(self._h, name.encode(), int(value)), "ifdm_set_option")
 
     def set_profiling(self, on: bool):
         self._chk(lib().ifdm_set_profiling(self._h, int(on)), "ifdm_set_profiling")

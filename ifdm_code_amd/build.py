@@ -13,8 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libifdm.so")
 ARCH = os.environ.get("IFDM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["ifdm_capi.hip", "cosmo.cpp"]
-HEADERS = ["fft_core.h", "kdk_passes.h", "constants.h"]
+SOURCES = ["ifdm_capi.hip", "ifdm_io.hip", "cosmo.cpp"]
+HEADERS = ["fft_core.h", "kdk_passes.h", "constants.h", "ctx.h", "fastmath.h", "ic_kernels.h"]
 
 
 def hipcc() -> str:
@@ -33,6 +33,15 @@ def command(out: str = LIB) -> list[str]:
             "-o", out] + srcs
 
 
+CLI = os.path.join(HERE, "ifdm")
+
+
+def cli_command(out: str = CLI) -> list[str]:
+    """The command-line driver: plain C++ over the C ABI."""
+    return ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "ifdm_main.cpp"), "-o", out,
+            "-L" + HERE, "-lifdm", "-Wl,-rpath,$ORIGIN"]
+
+
 def build(verbose: bool = True) -> str:
     tmp = LIB + ".tmp"
     cmd = command(tmp)
@@ -43,6 +52,13 @@ def build(verbose: bool = True) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libifdm.so")
     os.replace(tmp, LIB)
+    cmd = cli_command()
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building the ifdm driver")
     return LIB
 
 

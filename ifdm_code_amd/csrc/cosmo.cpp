@@ -172,6 +172,55 @@ extern "C" int ifdm_timestep_criteria(const ifdm_config *cfg, double a, const if
     return IFDM_OK;
 }
 
+// The snapshot trigger chain of Simulation (src/CosmoBEC_MPI_stan.cpp:238-284;
+// variant 1: src/CosmoBEC_MPI.cpp, whose chain differs at :250 and :256-261).
+// Evaluated after Timestep_criteria has advanced a (par.ai).
+extern "C" int ifdm_output_trigger(int variant, int a_triggers, double a, int tn, int twrite, int t_stop,
+                                   ifdm_trigger_state *st, int *t_out, int *stop) {
+    if (!st || !t_out || !stop || twrite <= 0) return -1;
+    *stop = 0;
+    *t_out = -1;
+    if (a_triggers) {
+        if (a > 1) {                                                   // :238
+            *t_out = 999999;
+            st->write_num++;
+            *stop = 1;
+            return 1;
+        } else if (a > 0.714 && st->write_num == 4) {                  // :244
+            *t_out = 999998;
+            st->write_num++;
+            return 1;
+        } else if (a > 0.5 && st->write_num == (variant == 1 ? 3 : 2)) {  // :250
+            *t_out = 999997;
+            st->write_num++;
+            *stop = 1;
+            return 1;
+        } else if (variant == 1 && a > 0.333 && st->write_num == 2) {  // CosmoBEC_MPI.cpp:256
+            *t_out = 999996;
+            st->write_num++;
+            return 1;
+        } else if (a > 0.25 && st->write_num == 2) {                   // :262
+            *t_out = st->write_sten_counter;
+            st->write_sten_counter++;
+            return 1;
+        } else if (a > 0.25 && st->write_num == 1) {                   // :269
+            *t_out = 999995;
+            st->write_num++;
+            st->sten_counter = 0;
+            return 1;
+        } else if (a > 0.105 && st->write_num == 0) {                  // :276
+            *t_out = 999994;
+            st->write_num++;
+            return 1;
+        }
+    }
+    if ((tn % twrite == 0) || tn == t_stop - 1) {                      // :282
+        *t_out = tn + 1;
+        return 1;
+    }
+    return 0;
+}
+
 extern "C" double ifdm_constant(const char *name) {
     if (!name) return std::nan("");
     struct { const char *k; double v; } tab[] = {

@@ -1,0 +1,100 @@
+// ctx.h -- the library context (opaque ifdm_ctx of include/ifdm.h) and the
+// error / profiling helpers shared by the translation units of libifdm.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ifdm.h"
+
+struct ifdm_ctx {
+    ifdm_config cfg{};
+    int n = 0;
+    long n3 = 0;
+    int hs = 0;
+    hipStream_t stream = nullptr;
+    void *psi = nullptr;       // N^3 complex
+    void *phi = nullptr;       // N^3 real
+    void *shat = nullptr;      // N*N*HS complex
+    void *tw = nullptr;        // N complex twiddles
+    double *k2 = nullptr;      // N doubles
+    double *part_rho = nullptr;
+    double *part_phi = nullptr;
+    double *red = nullptr;     // 3 doubles on device
+    double *red_stage = nullptr;  // 3 x IFDM_RED_BLOCKS doubles
+    void *dtab = nullptr;      // N complex: per-step drift table
+    double *red_host = nullptr;  // 3 doubles, pinned
+    int nz_blocks = 0;
+    // physics
+    double m_over_hbar = 0;    // (mfdm / c^2) / hbar
+    double g_over_hbar = 0;
+    double ms = 0;             // mfdm_solar
+    double ke_coef = 0;
+    double green = 0;
+    double fftnorm = 0;
+    // mixed precision: psi is stored as 2^-psi_scale psi (complex64)
+    bool mixed = false;
+    int psi_scale = 0;
+    double psc2 = 1.0;         // 2^(2 psi_scale)
+    double rho_mean = 0.0;     // m_s <|psi|^2>, FP64, subtracted in the mixed-mode source
+    // second half-kick of the last ifdm_step, not yet applied
+    bool pending = false;
+    double pend_aphi = 0.0, pend_arho = 0.0;
+    // initial-condition grids (ifdm_ic_begin .. ifdm_ic_finish), FP64
+    double *ic_rho = nullptr, *ic_vx = nullptr, *ic_vy = nullptr, *ic_vz = nullptr;
+    unsigned long long *ic_skipped = nullptr;
+    double ic_box = 0.0, ic_pmass = 0.0;
+    long long ic_npart = 0;
+    // launch schedule: 0 = one launch per pass; S > 0 = the x-slab-local
+    // chains (A,B), (D,E,F), (H,I) run slab by slab, S x-planes per slab
+    int slab = 0;
+    // optional live per-pass timing (ifdm_set_profiling)
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;          // 2 events per profiled launch
+    std::vector<int> ev_pass;                 // pass id of each recorded launch
+    int cur_pass = 0;                         // pass id of the launches being issued
+    double pass_ms[IFDM_NPASS] = {};
+    long pass_count = 0;
+    std::string err;
+};
+
+static constexpr int IFDM_RED_BLOCKS = 256;
+
+#define HIPCHK(ctx, expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                \
+            return IFDM_E_HIP;                                                             \
+        }                                                                                  \
+    } while (0)
+
+#define LAUNCHCHK(ctx) HIPCHK(ctx, hipGetLastError())
+
+static constexpr int IFDM_EV_POOL = 4096;
+
+// With live profiling on, brackets the next launch with a pair of events
+// from the pool and remembers its pass id.
+static inline int prof_begin(ifdm_ctx *c) {
+    if (!c->prof) return 0;
+    const size_t k = c->ev_pass.size();
+    if (2 * k + 1 >= c->ev_pool.size()) { c->err = "profiling event pool exhausted"; return IFDM_E_STATE; }
+    HIPCHK(c, hipEventRecord(c->ev_pool[2 * k], c->stream));
+    c->ev_pass.push_back(c->cur_pass);
+    return 0;
+}
+static inline int prof_end(ifdm_ctx *c) {
+    if (!c->prof) return 0;
+    HIPCHK(c, hipEventRecord(c->ev_pool[2 * (c->ev_pass.size() - 1) + 1], c->stream));
+    return 0;
+}
+
+
+// applies a pending second half-kick (defined in ifdm_capi.hip)
+extern "C" __attribute__((visibility("hidden"))) int ifdm_internal_flush(ifdm_ctx *c);
+
+// theta solve for the initial conditions: in-place forward 3-D FFT of the
+// FP64 complex field buf, multiply by coef / k^2 / N^3 (k = 0 -> 0),
+// inverse FFT (src/gadget_IC.h:222-244).  Defined in ifdm_capi.hip.
+extern "C" __attribute__((visibility("hidden"))) int ifdm_internal_theta_solve(ifdm_ctx *c, void *buf64, double coef);

@@ -12,58 +12,12 @@
 
 #include "../../include/ifdm.h"
 #include "constants.h"
+#include "ctx.h"
 #include "kdk_passes.h"
 
 using namespace ifdm;
 
-struct ifdm_ctx {
-    ifdm_config cfg{};
-    int n = 0;
-    long n3 = 0;
-    int hs = 0;
-    hipStream_t stream = nullptr;
-    void *psi = nullptr;       // N^3 complex
-    void *phi = nullptr;       // N^3 real
-    void *shat = nullptr;      // N*N*HS complex
-    void *tw = nullptr;        // N complex twiddles
-    double *k2 = nullptr;      // N doubles
-    double *part_rho = nullptr;
-    double *part_phi = nullptr;
-    double *red = nullptr;     // 3 doubles on device
-    double *red_host = nullptr;  // 3 doubles, pinned
-    int nz_blocks = 0;
-    // physics
-    double m_over_hbar = 0;    // (mfdm / c^2) / hbar
-    double g_over_hbar = 0;
-    double ms = 0;             // mfdm_solar
-    double ke_coef = 0;
-    double green = 0;
-    double fftnorm = 0;
-    // mixed precision: psi is stored as 2^-psi_scale psi (complex64)
-    bool mixed = false;
-    int psi_scale = 0;
-    double psc2 = 1.0;         // 2^(2 psi_scale)
-    double rho_mean = 0.0;     // m_s <|psi|^2>, FP64, subtracted in the mixed-mode source
-    // optional live per-pass timing (ifdm_set_profiling)
-    bool prof = false;
-    hipEvent_t ev[IFDM_NPASS + 1] = {};
-    double pass_ms[IFDM_NPASS] = {};
-    long pass_count = 0;
-    std::string err;
-};
-
 static thread_local std::string g_create_err;
-
-#define HIPCHK(ctx, expr)                                                                  \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess) {                                                            \
-            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                \
-            return IFDM_E_HIP;                                                             \
-        }                                                                                  \
-    } while (0)
-
-#define LAUNCHCHK(ctx) HIPCHK(ctx, hipGetLastError())
 
 namespace {
 
@@ -78,10 +32,16 @@ struct Ops {
         z.ms = c->ms * c->psc2;
         z.rhoc = c->mixed ? c->rho_mean : c->cfg.rhoc;
         z.ref_range = c->cfg.ref_maxphi_range;
+        z.blk0 = 0;
+        z.dtab = nullptr;
+        z.k2 = c->k2;
+        z.ke_dke = 0.0;
         return z;
     }
     static SArgs sargs(ifdm_ctx *c, double dKE) {
         SArgs s;
+        s.other0 = 0;
+        s.dtab = c->dtab;
         s.k2 = c->k2;
         s.ke_coef = c->ke_coef;
         s.dKE = dKE;
@@ -90,32 +50,49 @@ struct Ops {
         return s;
     }
 
+    // z pass over x-planes [x0, x0 + nx) (all planes when nx == 0)
     template <int MODE>
-    static int zpass(ifdm_ctx *c, const ZArgs &za, double *part) {
-        const int nb = (N * N + z_lines<N>() - 1) / z_lines<N>();
-        hipLaunchKernelGGL((k_zpass<T, N, MODE>), dim3(nb), dim3(z_lines<N>() * P), 0, c->stream,
+    static int zpass(ifdm_ctx *c, ZArgs za, double *part, int x0 = 0, int nx = 0) {
+        constexpr int LZ = z_lines<N>();
+        if (nx == 0) nx = N;
+        const long l0 = (long)x0 * N, nl = (long)nx * N;
+        za.blk0 = (int)(l0 / LZ);
+        const int nb = (int)((nl + LZ - 1) / LZ);
+        int rc;
+        if ((rc = prof_begin(c))) return rc;
+        hipLaunchKernelGGL((k_zpass<T, N, MODE>), dim3(nb), dim3(LZ * P), 0, c->stream,
                            (cx<T> *)c->psi, (T *)c->phi, (cx<T> *)c->shat, (const cx<T> *)c->tw, za, part);
         LAUNCHCHK(c);
-        return 0;
+        return prof_end(c);
     }
+    // strided pass; for AXIS 1 the fixed index (x) runs over [o0, o0 + no)
     template <int AXIS, int MODE>
-    static int spass(ifdm_ctx *c, void *arr, int rl, int kv, const SArgs &sa) {
+    static int spass(ifdm_ctx *c, void *arr, int rl, int kv, SArgs sa, int o0 = 0, int no = 0) {
         constexpr int TC = s_cols<T>();
-        dim3 grid((kv + TC - 1) / TC, N);
+        if (no == 0) no = N;
+        sa.other0 = o0;
+        dim3 grid((kv + TC - 1) / TC, no);
+        int rc;
+        if ((rc = prof_begin(c))) return rc;
         hipLaunchKernelGGL((k_spass<T, N, AXIS, MODE>), grid, dim3(TC * P), 0, c->stream,
                            (cx<T> *)arr, rl, kv, (const cx<T> *)c->tw, sa);
         LAUNCHCHK(c);
-        return 0;
+        return prof_end(c);
     }
 
     // One pass of the step by index: 0..8 = A..I, 9 = final reduction.
     // (aphi, arho) is the kick of pass A (both 0: no kick); dKE the drift of C.
     static int pass(ifdm_ctx *c, int idx, double aphi, double arho, double dKE) {
         const SArgs sa = sargs(c, dKE);
+        c->cur_pass = idx;
         switch (idx) {
-            case 0:
-                if (aphi != 0.0 || arho != 0.0) return zpass<Z_KICK_FWD>(c, zargs(c, aphi, arho), nullptr);
-                return zpass<Z_FWD>(c, zargs(c, 0, 0), nullptr);
+            case 0: {
+                ZArgs za = zargs(c, aphi, arho);
+                za.dtab = c->dtab;
+                za.ke_dke = c->ke_coef * dKE;
+                if (aphi != 0.0 || arho != 0.0) return zpass<Z_KICK_FWD>(c, za, nullptr);
+                return zpass<Z_FWD>(c, za, nullptr);
+            }
             case 1: return spass<1, S_FWD>(c, c->psi, N, N, sa);
             case 2: return spass<0, S_DRIFT>(c, c->psi, N, N, sa);
             case 3: return spass<1, S_INV>(c, c->psi, N, N, sa);
@@ -124,11 +101,17 @@ struct Ops {
             case 6: return spass<0, S_GREEN>(c, c->shat, c->hs, N / 2 + 1, sa);
             case 7: return spass<1, S_INV>(c, c->shat, c->hs, N / 2 + 1, sa);
             case 8: return zpass<Z_C2R>(c, zargs(c, 0, 0), c->part_phi);
-            case 9:
-                hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, c->stream, c->part_rho, c->nz_blocks,
-                                   c->part_phi, c->nz_blocks, c->red);
+            case 9: {
+                int rc;
+                if ((rc = prof_begin(c))) return rc;
+                hipLaunchKernelGGL(k_reduce1, dim3(IFDM_RED_BLOCKS), dim3(256), 0, c->stream, c->part_rho,
+                                   c->nz_blocks, c->part_phi, c->nz_blocks, c->red_stage);
                 LAUNCHCHK(c);
-                return 0;
+                hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, c->stream, c->red_stage, IFDM_RED_BLOCKS,
+                                   c->red);
+                LAUNCHCHK(c);
+                return prof_end(c);
+            }
             default: return IFDM_E_ARG;
         }
     }
@@ -141,22 +124,76 @@ struct Ops {
         return 0;
     }
 
+    // The x-slab-local chains run slab by slab so that the second pass of a
+    // chain reads what the first just wrote while it is still in the
+    // memory-side cache: (A, B) ... C ... (D, E, F) ... G ... (H, I).
+    static int kick_drift_poisson_slabs(ifdm_ctx *c, double aphi, double arho, double dKE) {
+        const int S = c->slab;
+        const SArgs sa = sargs(c, dKE);
+        ZArgs z0 = zargs(c, 0, 0);
+        ZArgs za = zargs(c, aphi, arho);
+        za.dtab = c->dtab;
+        za.ke_dke = c->ke_coef * dKE;
+        const bool kick = aphi != 0.0 || arho != 0.0;
+        int rc;
+        for (int x0 = 0; x0 < N; x0 += S) {
+            const int nx = std::min(S, N - x0);
+            c->cur_pass = 0;
+            rc = kick ? zpass<Z_KICK_FWD>(c, za, nullptr, x0, nx) : zpass<Z_FWD>(c, za, nullptr, x0, nx);
+            c->cur_pass = 1;
+            if (rc || (rc = spass<1, S_FWD>(c, c->psi, N, N, sa, x0, nx))) return rc;
+        }
+        c->cur_pass = 2;
+        if ((rc = spass<0, S_DRIFT>(c, c->psi, N, N, sa))) return rc;
+        for (int x0 = 0; x0 < N; x0 += S) {
+            const int nx = std::min(S, N - x0);
+            c->cur_pass = 3;
+            if ((rc = spass<1, S_INV>(c, c->psi, N, N, sa, x0, nx))) return rc;
+            c->cur_pass = 4;
+            if ((rc = zpass<Z_INV_SRC>(c, z0, c->part_rho, x0, nx))) return rc;
+            c->cur_pass = 5;
+            if ((rc = spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sa, x0, nx))) return rc;
+        }
+        c->cur_pass = 6;
+        if ((rc = spass<0, S_GREEN>(c, c->shat, c->hs, N / 2 + 1, sa))) return rc;
+        for (int x0 = 0; x0 < N; x0 += S) {
+            const int nx = std::min(S, N - x0);
+            c->cur_pass = 7;
+            if ((rc = spass<1, S_INV>(c, c->shat, c->hs, N / 2 + 1, sa, x0, nx))) return rc;
+            c->cur_pass = 8;
+            if ((rc = zpass<Z_C2R>(c, z0, c->part_phi, x0, nx))) return rc;
+        }
+        return pass(c, 9, 0, 0, 0);
+    }
+
     // passes A..I + reduction: kick (aphi, arho), drift dKE, Poisson.
     // With profiling on, each pass is bracketed by events on the stream.
     static int kick_drift_poisson(ifdm_ctx *c, double aphi, double arho, double dKE) {
+        if (c->slab > 0 && c->slab < N && ((long)c->slab * N) % z_lines<N>() == 0)
+            return kick_drift_poisson_slabs(c, aphi, arho, dKE);
         int rc;
-        for (int i = 0; i < IFDM_NPASS; ++i) {
-            if (c->prof) HIPCHK(c, hipEventRecord(c->ev[i], c->stream));
+        for (int i = 0; i < IFDM_NPASS; ++i)
             if ((rc = pass(c, i, aphi, arho, dKE))) return rc;
-        }
-        if (c->prof) HIPCHK(c, hipEventRecord(c->ev[IFDM_NPASS], c->stream));
         return 0;
+    }
+
+    // IC phase: forward FFT, coef/k^2/N^3 (k = 0 -> 0), inverse FFT of c->psi
+    static int theta_solve(ifdm_ctx *c, double coef) {
+        SArgs sa = sargs(c, 0.0);
+        sa.green = coef;
+        const ZArgs z0 = zargs(c, 0, 0);
+        int rc;
+        if ((rc = zpass<Z_FWD>(c, z0, nullptr))) return rc;
+        if ((rc = spass<1, S_FWD>(c, c->psi, N, N, sa))) return rc;
+        if ((rc = spass<0, S_GREEN>(c, c->psi, N, N, sa))) return rc;
+        if ((rc = spass<1, S_INV>(c, c->psi, N, N, sa))) return rc;
+        return zpass<Z_INV>(c, z0, nullptr);
     }
 
     static int drift(ifdm_ctx *c, double dKE) {
         SArgs sa = sargs(c, dKE);
         int rc;
-        if ((rc = zpass<Z_FWD>(c, zargs(c, 0, 0), nullptr))) return rc;
+        if ((rc = pass(c, 0, 0.0, 0.0, dKE))) return rc;
         if ((rc = spass<1, S_FWD>(c, c->psi, N, N, sa))) return rc;
         if ((rc = spass<0, S_DRIFT>(c, c->psi, N, N, sa))) return rc;
         if ((rc = spass<1, S_INV>(c, c->psi, N, N, sa))) return rc;
@@ -187,6 +224,7 @@ struct Ops {
 };
 
 struct Dispatch {
+    int (*theta_solve)(ifdm_ctx *, double);
     int (*pass)(ifdm_ctx *, int, double, double, double);
     int (*poisson)(ifdm_ctx *);
     int (*kdp)(ifdm_ctx *, double, double, double);
@@ -197,7 +235,7 @@ struct Dispatch {
 
 template <typename T, int N>
 Dispatch make_dispatch() {
-    return Dispatch{&Ops<T, N>::pass, &Ops<T, N>::poisson, &Ops<T, N>::kick_drift_poisson, &Ops<T, N>::drift, &Ops<T, N>::fft3d,
+    return Dispatch{&Ops<T, N>::theta_solve, &Ops<T, N>::pass, &Ops<T, N>::poisson, &Ops<T, N>::kick_drift_poisson, &Ops<T, N>::drift, &Ops<T, N>::fft3d,
                     &Ops<T, N>::kick};
 }
 
@@ -229,15 +267,19 @@ bool get_dispatch(int n, int precision, Dispatch *d) {
     }
 }
 
-// adds the elapsed time of each pass of the last recorded step
+// adds the elapsed time of every launch recorded since the last call to
+// its pass and counts one step
 int accumulate_profile(ifdm_ctx *c) {
     if (!c->prof) return 0;
-    HIPCHK(c, hipEventSynchronize(c->ev[IFDM_NPASS]));
-    for (int p = 0; p < IFDM_NPASS; ++p) {
+    const size_t k = c->ev_pass.size();
+    if (k == 0) return 0;
+    HIPCHK(c, hipEventSynchronize(c->ev_pool[2 * k - 1]));
+    for (size_t i = 0; i < k; ++i) {
         float ms = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[p], c->ev[p + 1]));
-        c->pass_ms[p] += ms;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_pool[2 * i], c->ev_pool[2 * i + 1]));
+        c->pass_ms[c->ev_pass[i]] += ms;
     }
+    c->ev_pass.clear();
     c->pass_count++;
     return 0;
 }
@@ -265,9 +307,12 @@ void release(ifdm_ctx *c) {
     if (!c) return;
     hipSetDevice(c->cfg.device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    for (auto &e : c->ev)
+    for (auto &e : c->ev_pool)
         if (e) hipEventDestroy(e);
-    void *bufs[] = {c->psi, c->phi, c->shat, c->tw, c->k2, c->part_rho, c->part_phi, c->red};
+    void *icb[] = {c->ic_rho, c->ic_vx, c->ic_vy, c->ic_vz, c->ic_skipped};
+    for (void *b : icb)
+        if (b) hipFree(b);
+    void *bufs[] = {c->psi, c->phi, c->shat, c->tw, c->k2, c->part_rho, c->part_phi, c->red, c->red_stage, c->dtab};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (c->red_host) hipHostFree(c->red_host);
@@ -276,6 +321,19 @@ void release(ifdm_ctx *c) {
 }
 
 bool pow2_in_range(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
+
+// twiddles W_N^k = exp(-2 pi i k / N), k < N, evaluated in long double
+void host_twiddles(int n, std::vector<double> &tw, std::vector<float> &twf) {
+    tw.resize(2 * (size_t)n);
+    twf.resize(2 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+        tw[2 * k] = (double)cosl(a);
+        tw[2 * k + 1] = (double)-sinl(a);
+        twf[2 * k] = (float)cosl(a);
+        twf[2 * k + 1] = (float)-sinl(a);
+    }
+}
 
 }  // namespace
 
@@ -328,6 +386,8 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
         {(void **)&c->part_rho, 2 * ds * (size_t)c->nz_blocks},
         {(void **)&c->part_phi, ds * (size_t)c->nz_blocks},
         {(void **)&c->red, 3 * ds},
+        {(void **)&c->red_stage, 3 * ds * IFDM_RED_BLOCKS},
+        {&c->dtab, cs * (size_t)n},
     };
     for (auto &a : allocs) {
         if ((e = hipMalloc(a.p, a.bytes)) != hipSuccess)
@@ -336,16 +396,10 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     if ((e = hipHostMalloc((void **)&c->red_host, 3 * ds)) != hipSuccess)
         return fail(IFDM_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
     // twiddles W_N^k = exp(-2 pi i k / N) evaluated in long double
-    std::vector<double> tw(2 * (size_t)n), k2(n);
-    std::vector<float> twf(2 * (size_t)n);
-    for (int k = 0; k < n; ++k) {
-        const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
-        tw[2 * k] = (double)cosl(a);
-        tw[2 * k + 1] = (double)-sinl(a);
-        twf[2 * k] = (float)cosl(a);
-        twf[2 * k + 1] = (float)-sinl(a);
-        k2[k] = std::pow(kvec(k, n, cfg->xmax), 2);
-    }
+    std::vector<double> tw, k2(n);
+    std::vector<float> twf;
+    host_twiddles(n, tw, twf);
+    for (int k = 0; k < n; ++k) k2[k] = std::pow(kvec(k, n, cfg->xmax), 2);
     const void *twsrc = c->mixed ? (const void *)twf.data() : (const void *)tw.data();
     if ((e = hipMemcpy(c->tw, twsrc, cs * n, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(c->k2, k2.data(), ds * n, hipMemcpyHostToDevice)) != hipSuccess ||
@@ -376,6 +430,7 @@ void ifdm_destroy(ifdm_ctx *ctx) { release(ctx); }
 int ifdm_set_psi(ifdm_ctx *c, const double *psi) {
     if (!c || !psi) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    c->pending = false;
     if (!c->mixed) {
         HIPCHK(c, hipMemcpyAsync(c->psi, psi, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -395,6 +450,7 @@ int ifdm_set_psi(ifdm_ctx *c, const double *psi) {
 int ifdm_get_psi(ifdm_ctx *c, double *psi) {
     if (!c || !psi) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (int rc0 = ifdm_internal_flush(c)) return rc0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (!c->mixed) {
         HIPCHK(c, hipMemcpy(psi, c->psi, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToHost));
@@ -423,6 +479,7 @@ int ifdm_get_phi(ifdm_ctx *c, double *phi) {
 int ifdm_set_phi(ifdm_ctx *c, const double *phi) {
     if (!c || !phi) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (int rc0 = ifdm_internal_flush(c)) return rc0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (!c->mixed) {
         HIPCHK(c, hipMemcpy(c->phi, phi, sizeof(double) * (size_t)c->n3, hipMemcpyHostToDevice));
@@ -441,7 +498,9 @@ int ifdm_poisson(ifdm_ctx *c, ifdm_stats *st) {
     HIPCHK(c, hipSetDevice(c->cfg.device));
     int rc = d.poisson(c);
     if (rc) return rc;
-    return read_stats(c, st);
+    rc = read_stats(c, st);
+    c->ev_pass.clear();  // a lone Poisson solve is not a profiled step
+    return rc;
 }
 
 int ifdm_kick(ifdm_ctx *c, double dPE, double dQI) {
@@ -449,6 +508,7 @@ int ifdm_kick(ifdm_ctx *c, double dPE, double dQI) {
     Dispatch d;
     get_dispatch(c->n, c->cfg.precision, &d);
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (int rc0 = ifdm_internal_flush(c)) return rc0;
     int rc = d.kick(c, 0.5 * dPE * c->m_over_hbar, 0.5 * dQI * c->g_over_hbar);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -460,6 +520,7 @@ int ifdm_drift(ifdm_ctx *c, double dKE) {
     Dispatch d;
     get_dispatch(c->n, c->cfg.precision, &d);
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (int rc0 = ifdm_internal_flush(c)) return rc0;
     int rc = d.drift(c, dKE);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -471,6 +532,7 @@ int ifdm_fft3d(ifdm_ctx *c, int sign) {
     Dispatch d;
     get_dispatch(c->n, c->cfg.precision, &d);
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (int rc0 = ifdm_internal_flush(c)) return rc0;
     int rc = d.fft3d(c, sign);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -482,43 +544,66 @@ int ifdm_kdk_step(ifdm_ctx *c, const ifdm_dt *dt, ifdm_stats *st) {
     Dispatch d;
     get_dispatch(c->n, c->cfg.precision, &d);
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (int rc0 = ifdm_internal_flush(c)) return rc0;
     const double aphi = 0.5 * dt->dPE * c->m_over_hbar, arho = 0.5 * dt->dQI * c->g_over_hbar;
     int rc = d.kdp(c, aphi, arho, dt->dKE);
     if (rc) return rc;
     if ((rc = d.kick(c, aphi, arho))) return rc;
-    return read_stats(c, st);
+    if ((rc = read_stats(c, st))) return rc;
+    return accumulate_profile(c);
 }
 
-int ifdm_run(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, ifdm_stats *st) {
-    if (!c || (!sched && nsteps > 0) || nsteps < 0) return IFDM_E_ARG;
+int ifdm_step(ifdm_ctx *c, const ifdm_dt *dt, ifdm_stats *st) {
+    if (!c || !dt) return IFDM_E_ARG;
     Dispatch d;
     get_dispatch(c->n, c->cfg.precision, &d);
     HIPCHK(c, hipSetDevice(c->cfg.device));
-    double pphi = 0.0, prho = 0.0;  // pending second half-kick of the previous step
-    int rc;
-    for (int s = 0; s < nsteps; ++s) {
-        const double aphi = 0.5 * sched[s].dPE * c->m_over_hbar, arho = 0.5 * sched[s].dQI * c->g_over_hbar;
-        if ((rc = d.kdp(c, pphi + aphi, prho + arho, sched[s].dKE))) return rc;
-        pphi = aphi;
-        prho = arho;
-        if (st && (rc = read_stats(c, &st[s]))) return rc;
-        if ((rc = accumulate_profile(c))) return rc;
+    const double aphi = 0.5 * dt->dPE * c->m_over_hbar, arho = 0.5 * dt->dQI * c->g_over_hbar;
+    double kphi = aphi, krho = arho;
+    if (c->pending) {
+        kphi += c->pend_aphi;
+        krho += c->pend_arho;
     }
-    if (nsteps > 0 && (rc = d.kick(c, pphi, prho))) return rc;
+    int rc;
+    if ((rc = d.kdp(c, kphi, krho, dt->dKE))) return rc;
+    c->pending = true;
+    c->pend_aphi = aphi;
+    c->pend_arho = arho;
+    if (st && (rc = read_stats(c, st))) return rc;
+    return accumulate_profile(c);
+}
+
+int ifdm_internal_flush(ifdm_ctx *c) {
+    if (!c->pending) return IFDM_OK;
+    Dispatch d;
+    get_dispatch(c->n, c->cfg.precision, &d);
+    int rc;
+    if ((rc = d.kick(c, c->pend_aphi, c->pend_arho))) return rc;
+    c->pending = false;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return IFDM_OK;
 }
 
+int ifdm_flush(ifdm_ctx *c) {
+    if (!c) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    return ifdm_internal_flush(c);
+}
+
+int ifdm_run(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, ifdm_stats *st) {
+    if (!c || (!sched && nsteps > 0) || nsteps < 0) return IFDM_E_ARG;
+    int rc;
+    for (int s = 0; s < nsteps; ++s)
+        if ((rc = ifdm_step(c, &sched[s], st ? &st[s] : nullptr))) return rc;
+    return ifdm_flush(c);
+}
+
 int ifdm_simulate(ifdm_ctx *c, double a0, int nsteps, double *a_out, ifdm_dt *sched_out) {
     if (!c || nsteps < 0 || !(a0 > 0.0)) return IFDM_E_ARG;
-    Dispatch d;
-    get_dispatch(c->n, c->cfg.precision, &d);
-    HIPCHK(c, hipSetDevice(c->cfg.device));
     int rc;
-    if ((rc = d.poisson(c))) return rc;                            // :218 of the first step
     ifdm_stats st;
-    if ((rc = read_stats(c, &st))) return rc;
-    double a = a0, pphi = 0.0, prho = 0.0;
+    if ((rc = ifdm_poisson(c, &st))) return rc;                  // :218 of the first step
+    double a = a0;
     for (int s = 0; s < nsteps; ++s) {
         ifdm_dt dt;
         double an;
@@ -528,17 +613,47 @@ int ifdm_simulate(ifdm_ctx *c, double a0, int nsteps, double *a_out, ifdm_dt *sc
         }
         if (sched_out) sched_out[s] = dt;
         a = an;
-        const double aphi = 0.5 * dt.dPE * c->m_over_hbar, arho = 0.5 * dt.dQI * c->g_over_hbar;
-        if ((rc = d.kdp(c, pphi + aphi, prho + arho, dt.dKE))) return rc;
-        pphi = aphi;
-        prho = arho;
-        if ((rc = read_stats(c, &st))) return rc;
-        if ((rc = accumulate_profile(c))) return rc;
+        if ((rc = ifdm_step(c, &dt, &st))) return rc;
     }
-    if (nsteps > 0 && (rc = d.kick(c, pphi, prho))) return rc;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = ifdm_flush(c))) return rc;
     if (a_out) *a_out = a;
     return IFDM_OK;
+}
+
+int ifdm_internal_theta_solve(ifdm_ctx *c, void *buf64, double coef) {
+    Dispatch d;
+    get_dispatch(c->n, IFDM_FP64, &d);
+    if (!c->mixed && buf64 == c->psi) {
+        int rc = d.theta_solve(c, coef);
+        if (rc) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return IFDM_OK;
+    }
+    // mixed mode: run the FP64 passes on a shadow context over buf64
+    ifdm_ctx sh;
+    sh.cfg = c->cfg;
+    sh.cfg.precision = IFDM_FP64;
+    sh.n = c->n;
+    sh.n3 = c->n3;
+    sh.hs = c->hs;
+    sh.stream = c->stream;
+    sh.psi = buf64;
+    sh.k2 = c->k2;
+    sh.ke_coef = c->ke_coef;
+    sh.green = c->green;
+    sh.fftnorm = c->fftnorm;
+    std::vector<double> tw;
+    std::vector<float> twf;
+    host_twiddles(c->n, tw, twf);
+    HIPCHK(c, hipMalloc(&sh.tw, tw.size() * sizeof(double)));
+    int rc = IFDM_OK;
+    if (hipMemcpy(sh.tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = IFDM_E_HIP;
+    if (!rc) rc = d.theta_solve(&sh, coef);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = IFDM_E_HIP;
+    if (rc && c->err.empty()) c->err = sh.err.empty() ? "theta solve failed" : sh.err;
+    hipFree(sh.tw);
+    sh.stream = nullptr;
+    return rc;
 }
 
 int ifdm_stream_copy(ifdm_ctx *c, size_t bytes, int iters, double *gbs) {
@@ -577,12 +692,26 @@ int ifdm_stream_copy(ifdm_ctx *c, size_t bytes, int iters, double *gbs) {
     return rc;
 }
 
+int ifdm_set_option(ifdm_ctx *c, const char *name, long value) {
+    if (!c || !name) return IFDM_E_ARG;
+    if (std::strcmp(name, "slab_planes") == 0) {
+        if (value < 0 || value > c->n) { c->err = "slab_planes out of range"; return IFDM_E_ARG; }
+        c->slab = (int)value;
+        return IFDM_OK;
+    }
+    c->err = std::string("unknown option ") + name;
+    return IFDM_E_ARG;
+}
+
 int ifdm_set_profiling(ifdm_ctx *c, int on) {
     if (!c) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
-    if (on && !c->ev[0])
-        for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
+    if (on && c->ev_pool.empty()) {
+        c->ev_pool.resize(IFDM_EV_POOL, nullptr);
+        for (auto &e : c->ev_pool) HIPCHK(c, hipEventCreate(&e));
+    }
     c->prof = on != 0;
+    c->ev_pass.clear();
     for (double &x : c->pass_ms) x = 0.0;
     c->pass_count = 0;
     return IFDM_OK;

@@ -22,6 +22,7 @@
 // poisson() at :80-119; its drift multiplies by fftnorm at :141 and its
 // Poisson solve by fftnorm after the c2r at :116.
 #pragma once
+#include "fastmath.h"
 #include "fft_core.h"
 
 namespace ifdm {
@@ -36,9 +37,15 @@ struct ZArgs {
     double rhoc;      // Poisson source: subtracted density (rho_c; the FP64
                       // mean m_s <|psi|^2> in mixed mode)
     int ref_range;    // max|phi| over the reference's one-rank scan range
+    int blk0;         // first block of this launch (a launch may cover one x-slab)
+    void *dtab;       // drift table to fill (block 0 of a forward z pass), or null
+    const double *k2; // k_i^2 for the drift table
+    double ke_dke;    // ke_coef * dKE for the drift table
 };
 
 struct SArgs {
+    int other0;        // first line index of the fixed axis in this launch (x-slab)
+    const void *dtab;  // drift table E[i] (pass C)
     const double *k2;  // k_i^2, i < N (src/struct3D.h:76-85 squared)
     double ke_coef;    // 0.5 hbar / (m / c^2)   (src/struct3D.h:121)
     double dKE;        // time integral of the drift
@@ -55,34 +62,37 @@ template <int N> constexpr int z_lines() {
 }
 template <typename T> constexpr int s_cols() { return 128 / (int)sizeof(cx<T>); }  // 128-byte rows
 
-// exp(-i th) as the storage precision: FP64 sincos, or (mixed mode) the
-// FP64 argument reduced mod 2 pi in FP64 (two-term 2 pi) and an FP32 sincos.
-template <typename T> __device__ __forceinline__ cx<T> expmi(double th);
-template <> __device__ __forceinline__ cx<double> expmi<double>(double th) {
-    double sn, cs;
-    sincos(th, &sn, &cs);
-    return mkc<double>(cs, -sn);
-}
-template <> __device__ __forceinline__ cx<float> expmi<float>(double th) {
-    const double k = rint(th * 0.15915494309189533577);
-    double r = fma(-k, 6.283185307179586232, th);
-    r = fma(-k, 2.4492935982947064e-16, r);
-    float sn, cs;
-    sincosf((float)r, &sn, &cs);
-    return mkc<float>(cs, -sn);
+// exp(-i th) in the storage precision T: the FP64 argument is reduced mod
+// pi/2 in FP64, then sin/cos are evaluated in T (fastmath.h)
+template <typename T> __device__ __forceinline__ cx<T> expmi(double th) {
+    cx<T> r;
+    expmi_parts<T>(th, &r.x, &r.y);
+    return r;
 }
 
-// block reduction over the whole work-group; sbuf holds >= nthreads doubles
+// The drift phase exp(-i KE(k) dKE), KE = ke_coef (kx^2 + ky^2 + kz^2), is a
+// product of one factor per axis.  One block of pass A writes the per-step
+// table E[i] = exp(-i ke_coef k_i^2 dKE), i < N, which pass C reads.
+template <typename T, int N>
+__device__ __forceinline__ void write_drift_table(cx<T> *tab, const double *k2, double ke_dke, int tid, int nt) {
+    for (int i = tid; i < N; i += nt) tab[i] = expmi<T>((ke_dke * k2[i]));
+}
+
+template <bool MAX> __device__ __forceinline__ double red_op(double a, double b) { return MAX ? fmax(a, b) : a + b; }
+
+// block reduction: butterfly within each 64-wide wavefront, then one LDS
+// exchange across the block's wavefronts (sbuf: >= 16 doubles)
 template <bool MAX>
 __device__ __forceinline__ double block_reduce(double v, double *sbuf, int tid, int nthreads) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = red_op<MAX>(v, __shfl_xor(v, o, 64));
+    const int nw = (nthreads + 63) / 64;
     __syncthreads();
-    sbuf[tid] = v;
+    if ((tid & 63) == 0) sbuf[tid >> 6] = v;
     __syncthreads();
-    for (int s = nthreads / 2; s > 0; s >>= 1) {
-        if (tid < s) sbuf[tid] = MAX ? fmax(sbuf[tid], sbuf[tid + s]) : sbuf[tid] + sbuf[tid + s];
-        __syncthreads();
-    }
-    return sbuf[0];
+    double r = sbuf[0];
+    for (int w = 1; w < nw; ++w) r = red_op<MAX>(r, sbuf[w]);
+    return r;
 }
 
 // ------------------------------------------------------------------ z passes
@@ -94,7 +104,8 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
     constexpr int HS = hs_of(N), NT = LZ * P;
     __shared__ cx<T> lds[LZ * LS];
     const int tid = threadIdx.x, l = tid / P, t = tid - l * P;
-    const long line = (long)blockIdx.x * LZ + l;
+    const int blk = za.blk0 + (int)blockIdx.x;
+    const long line = (long)blk * LZ + l;
     const bool valid = line < (long)N * N;
     cx<T> *L = lds + l * LS;
     cx<T> v[E];
@@ -125,7 +136,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         }
         double *sb = reinterpret_cast<double *>(lds);
         const double bm = block_reduce<true>(mx, sb, tid, NT);
-        if (tid == 0) part[blockIdx.x] = bm;
+        if (tid == 0) part[blk] = bm;
         return;
     } else {
         cx<T> *row = psi + line * N;
@@ -142,6 +153,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             }
         }
         if constexpr (MODE == Z_FWD || MODE == Z_KICK_FWD) {
+            if (za.dtab && blk == 0) write_drift_table<T, N>((cx<T> *)za.dtab, za.k2, za.ke_dke, tid, NT);
             fft_line<T, N, -1>(v, L, t, tw);
             if (valid) {
 #pragma unroll
@@ -177,7 +189,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             double *sb = reinterpret_cast<double *>(lds);
             const double bm = block_reduce<true>(mx, sb, tid, NT);
             const double bs = block_reduce<false>(sm, sb, tid, NT);
-            if (tid == 0) { part[2 * blockIdx.x] = bm; part[2 * blockIdx.x + 1] = bs; }
+            if (tid == 0) { part[2 * blk] = bm; part[2 * blk + 1] = bs; }
         }
     }
 }
@@ -193,7 +205,7 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     const int k = blockIdx.x * TC + c;
-    const int other = blockIdx.y;
+    const int other = sa.other0 + (int)blockIdx.y;
     const bool valid = k < kv;
     long base, stride;
     if constexpr (AXIS == 1) { base = (long)other * N * rl + k; stride = rl; }
@@ -208,15 +220,16 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
     else {
         fft_line<T, N, -1>(v, L, t, tw);
         // spectral indices of this thread's points: along the line i, fixed `other` and k
-        const double k2o = sa.k2[other] + 0.0, k2k = valid ? sa.k2[k] : 0.0;
+        if constexpr (MODE == S_DRIFT) {
+            const cx<T> *tab = (const cx<T> *)sa.dtab;
+            const cx<T> w = cscale(cmul(tab[other], tab[valid ? k : 0]), (T)sa.fftnorm);
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int i = t + e * P;
-            const double k2 = sa.k2[i] + k2o + k2k;
-            if constexpr (MODE == S_DRIFT) {
-                const cx<T> ph = expmi<T>((sa.ke_coef * k2) * sa.dKE);
-                v[e] = cmul(v[e], cscale(ph, (T)sa.fftnorm));
-            } else {
+            for (int e = 0; e < E; ++e) v[e] = cmul(v[e], cmul(tab[t + e * P], w));
+        } else {
+            const double k2o = sa.k2[other], k2k = valid ? sa.k2[k] : 0.0;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const double k2 = sa.k2[t + e * P] + k2o + k2k;
                 const double gf = (k2 == 0.0) ? 0.0 : sa.green / k2 * sa.fftnorm;
                 v[e] = cscale(v[e], (T)gf);
             }
@@ -246,18 +259,35 @@ __global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, d
         dst[i] = src[i];
 }
 
-// final reduction of the per-block partials: out = {max|phi|, max|psi|^2, sum|psi|^2}
-__global__ __launch_bounds__(1024) void k_reduce(const double *__restrict__ prho, int nrho,
+// final reduction of the per-block partials in two stages (latency-bound
+// otherwise): stage 1 with 256-thread blocks, each thread one partial per
+// stride; stage 2 one block.  out = {max|phi|, max|psi|^2, sum|psi|^2}.
+__global__ __launch_bounds__(256) void k_reduce1(const double *__restrict__ prho, int nrho,
                                                  const double *__restrict__ pphi, int nphi,
-                                                 double *__restrict__ out) {
-    __shared__ double sb[1024];
-    const int tid = threadIdx.x;
+                                                 double *__restrict__ stage) {
+    __shared__ double sb[16];
+    const int tid = threadIdx.x, g = blockIdx.x * 256 + tid, stride = gridDim.x * 256;
     double mr = 0.0, sm = 0.0, mp = 0.0;
-    for (int i = tid; i < nrho; i += 1024) { mr = fmax(mr, prho[2 * i]); sm += prho[2 * i + 1]; }
-    for (int i = tid; i < nphi; i += 1024) mp = fmax(mp, pphi[i]);
-    const double a = block_reduce<true>(mp, sb, tid, 1024);
-    const double b = block_reduce<true>(mr, sb, tid, 1024);
-    const double c = block_reduce<false>(sm, sb, tid, 1024);
+    for (int i = g; i < nrho; i += stride) { mr = fmax(mr, prho[2 * i]); sm += prho[2 * i + 1]; }
+    for (int i = g; i < nphi; i += stride) mp = fmax(mp, pphi[i]);
+    const double a = block_reduce<true>(mp, sb, tid, 256);
+    const double b = block_reduce<true>(mr, sb, tid, 256);
+    const double c = block_reduce<false>(sm, sb, tid, 256);
+    if (tid == 0) { stage[3 * blockIdx.x] = a; stage[3 * blockIdx.x + 1] = b; stage[3 * blockIdx.x + 2] = c; }
+}
+
+__global__ __launch_bounds__(256) void k_reduce2(const double *__restrict__ stage, int nst, double *__restrict__ out) {
+    __shared__ double sb[16];
+    const int tid = threadIdx.x;
+    double mp = 0.0, mr = 0.0, sm = 0.0;
+    for (int i = tid; i < nst; i += 256) {
+        mp = fmax(mp, stage[3 * i]);
+        mr = fmax(mr, stage[3 * i + 1]);
+        sm += stage[3 * i + 2];
+    }
+    const double a = block_reduce<true>(mp, sb, tid, 256);
+    const double b = block_reduce<true>(mr, sb, tid, 256);
+    const double c = block_reduce<false>(sm, sb, tid, 256);
     if (tid == 0) { out[0] = a; out[1] = b; out[2] = c; }
 }
 

@@ -103,6 +103,17 @@ int ifdm_kick(ifdm_ctx *ctx, double dPE, double dQI);
  * and psi (what the next Timestep_criteria reads); may be NULL. */
 int ifdm_kdk_step(ifdm_ctx *ctx, const ifdm_dt *dt, ifdm_stats *stats);
 
+/* One Split_op_MPI step with the second half-kick deferred: the first
+ * half-kick of this step is merged with the deferred half-kick of the
+ * previous ifdm_step (one phase multiply), then drift and Poisson; this
+ * step's second half-kick stays pending.  Every call that observes or
+ * changes psi from outside (ifdm_get_psi, ifdm_write_snapshot, ifdm_kick,
+ * ifdm_drift, ifdm_fft3d, ifdm_kdk_step, ifdm_set_phi) applies a pending
+ * half-kick first (ifdm_flush); ifdm_set_psi discards it.  stats: the
+ * post-step reductions (unchanged by the pending kick); may be NULL. */
+int ifdm_step(ifdm_ctx *ctx, const ifdm_dt *dt, ifdm_stats *stats);
+int ifdm_flush(ifdm_ctx *ctx);
+
 /* nsteps consecutive Split_op_MPI steps with a given schedule (replay of a
  * recorded run), merging the second half-kick of step n with the first of
  * step n+1 into one pass.  The state on return equals nsteps calls of
@@ -122,6 +133,13 @@ int ifdm_simulate(ifdm_ctx *ctx, double a0, int nsteps, double *a_out, ifdm_dt *
  * fused schedule, then the final reduction).  The state is advanced. */
 #define IFDM_NPASS 10
 int ifdm_time_passes(ifdm_ctx *ctx, const ifdm_dt *dt, int iters, float *ms_out);
+
+/* Launch-schedule options (no effect on results):
+ *   "slab_planes" S: 0 (default) launches each pass over the whole grid;
+ *   S > 0 runs the x-slab-local pass chains slab by slab, S x-planes per
+ *   slab, so that the second pass of a chain finds its input in the
+ *   memory-side cache.  Ignored while live profiling is on. */
+int ifdm_set_option(ifdm_ctx *ctx, const char *name, long value);
 
 /* Live per-pass timing: with on != 0 every step run by ifdm_run or
  * ifdm_simulate brackets each pass with HIP events on the context's stream
@@ -144,6 +162,52 @@ int ifdm_drift(ifdm_ctx *ctx, double dKE);
  * src/CosmoBEC_MPI_stan.cpp:136,144. */
 int ifdm_fft3d(ifdm_ctx *ctx, int sign);
 
+/* ---- initial conditions and snapshots -------------------------------- */
+
+/* What the Gadget-1 IC header gave (src/gadget_IC.h:289-329). */
+typedef struct {
+    double box;            /* BoxSize, kpc/h */
+    double pmass;          /* mass[1] * 1e10, Msun/h (:317) */
+    double a;              /* time: the scale factor of the IC (:318) */
+    double rhoc;           /* mean density derived from the header (:319), Msun/Mpc^3 */
+    long long npart;       /* particles read */
+    long long skipped;     /* particles outside [-Lg/2, box) in some coordinate, not deposited */
+} ifdm_ic_info;
+
+/* read_gadget, src/gadget_IC.h:573-672 at one rank: reads the nchunks
+ * files <icpath>.0 .. (4 in the reference, :610), checks each BoxSize
+ * against 2 xmax 1e3 h_0 (:309; IFDM_E_FORMAT instead of exit), deposits
+ * every particle with the 8-cell CIC on the device (CIC_scheme, :45-156),
+ * forms the mass-weighted velocities, the velocity-divergence phase
+ * (initial_velocity_Poisson_Solver, :157-247) and psi = 1.225 sqrt(rho/m_s)
+ * exp(i m theta / hbar) (:666), and sets the context's rho_c (par.rhoc).
+ * ref_ghosts = 1 also deposits the trailing particles of each x-sorted chunk
+ * beyond the last cell centre a second time, as the reference does at one
+ * rank (:445-448).  voutput_path (may be NULL): append the float vx grid
+ * there (write_binary_vg, src/write3D.h:24-53). */
+int ifdm_read_gadget(ifdm_ctx *ctx, const char *icpath, int nchunks, int ref_ghosts,
+                     const char *voutput_path, ifdm_ic_info *info);
+
+/* The same IC pipeline from particles already in host memory:
+ * ifdm_ic_begin (box kpc/h, particle mass Msun/h), any number of
+ * ifdm_ic_deposit calls (np particles, pos and vel as 3*np floats in kpc/h
+ * and Gadget velocity units), then ifdm_ic_finish with the scale factor.
+ * theta_out (N^3 complex, may be NULL) receives the phase field opr.phik,
+ * vx_out (N^3 floats, may be NULL) the mass-weighted vx grid. */
+int ifdm_ic_begin(ifdm_ctx *ctx, double box_kpc_h, double pmass_msun_h);
+int ifdm_ic_deposit(ifdm_ctx *ctx, const float *pos, const float *vel, long long np, int ref_ghosts);
+int ifdm_ic_finish(ifdm_ctx *ctx, double a, double *theta_out, float *vx_out, long long *skipped_out);
+
+/* rho_c subtracted in the Poisson source (par.rhoc; the rerun path sets
+ * the constant 3.51059e10, src/gadget_IC.h:744). */
+int ifdm_set_rhoc(ifdm_ctx *ctx, double rhoc);
+
+/* writeall (src/write3D.h:86-97): output file [f64 mfdm eV][f64 g eV cm^3]
+ * [f64 a][f64 x[N]][complex128 psi N^3]; psi is streamed plane by plane. */
+int ifdm_write_snapshot(ifdm_ctx *ctx, const char *path, double a, double mfdm_ev, double g_ev_cm3);
+/* read_rerun (src/gadget_IC.h:709-750): psi and a from such a file. */
+int ifdm_read_snapshot(ifdm_ctx *ctx, const char *path, double *a_out, double *mfdm_out, double *g_out);
+
 /* ---- host-side physics (no device work) ---------------------------- */
 
 /* Timestep_criteria, src/CosmoBEC_MPI_stan.cpp:19-78: from the reductions
@@ -152,6 +216,22 @@ int ifdm_fft3d(ifdm_ctx *ctx, int sign);
  * a_next.  which_out (may be NULL) receives 3, 2 or 4 as the reference's n. */
 int ifdm_timestep_criteria(const ifdm_config *cfg, double a, const ifdm_stats *stats,
                            ifdm_dt *dt_out, double *a_next, int *which_out);
+
+/* Snapshot triggers of the time loop (Simulation,
+ * src/CosmoBEC_MPI_stan.cpp:238-284; variant 1 = src/CosmoBEC_MPI.cpp,
+ * which differs at :250 and :256-261).  Called once per step after
+ * Timestep_criteria, with the updated a.  Returns 1 when a snapshot is due
+ * (its number in *t_out), 0 otherwise, -1 on bad arguments; *stop = 1 when
+ * the loop ends after this write (the reference's `break`).  a_triggers = 0
+ * keeps only the every-twrite / last-step trigger (:282).  The state starts
+ * as {0, 0, 2} (write_num, sten_counter, write_sten_counter, :208-210). */
+typedef struct {
+    int write_num;
+    int sten_counter;
+    int write_sten_counter;
+} ifdm_trigger_state;
+int ifdm_output_trigger(int variant, int a_triggers, double a, int tn, int twrite, int t_stop,
+                        ifdm_trigger_state *st, int *t_out, int *stop);
 
 /* integral of 1/(H(x) x^power) dx from a0 to a1 (Ha{2,3,4}_preintegrat,
  * src/struct3D.h:127-146) by the same adaptive Gauss-Kronrod rule. */

@@ -302,20 +302,29 @@ void oracle_set_threads(int t) {
 
 /* ------------------------------------------------------------- IC path */
 
+/* The reference accumulates the three velocity grids in float
+ * (vxg, vyg, vzg are std::vector<float>, src/gadget_IC.h:583) and forms the
+ * divergence in float (:170).  The oracle keeps them in double and, with
+ * f32 != 0, rounds every stored value to float exactly where the reference
+ * stores into a float, reproducing its arithmetic. */
+static inline double st32(double v, int f32) { return f32 ? (double)(float)v : v; }
+
 /* CIC_scheme, src/gadget_IC.h:45-156, at one rank (my_rank = 0,
- * local_n0 = N).  rho: N^3 doubles (Re of opr.wfc), vx/vy/vz: N^3 floats
- * (vxg, vyg, vzg are std::vector<float>, :583).  Positions in kpc/h,
- * box in kpc/h, pmass in Msun/h (header mass[1] * 1e10, :317).  xg[i] =
- * i*Lg + Lg/2 (:333). */
-static void cic_one(int Ng, double box, double pmass, double Lg, const double *xg,
-                    float xp, float yp, float zp, float vxp, float vyp, float vzp,
-                    double *rho, float *vx, float *vy, float *vz) {
+ * local_n0 = N).  rho: N^3 doubles (Re of opr.wfc).  Positions in kpc/h,
+ * box in kpc/h, pmass in Msun/h (header mass[1] * 1e10, :317), xg[i] =
+ * i*Lg + Lg/2 (:333).  Returns 0, or 1 when the particle lies outside
+ * [-Lg/2, box) in some coordinate and is skipped (the reference would index
+ * out of bounds there). */
+static int cic_one(int Ng, double box, double pmass, double Lg, const double *xg,
+                   float xp, float yp, float zp, float vxp, float vyp, float vzp,
+                   double *rho, double *vx, double *vy, double *vz, int f32) {
     int gi = (int)floor((xp - 0.5 * Lg) / Lg);
     int gj = (int)floor((yp - 0.5 * Lg) / Lg);
     int gk = (int)floor((zp - 0.5 * Lg) / Lg);
     if (gi == -1) { gi = Ng - 1; xp += box; }
     if (gj < 0) { gj = Ng - 1; yp += box; }
     if (gk < 0) { gk = Ng - 1; zp += box; }
+    if (gi < 0 || gi >= Ng || gj >= Ng || gk >= Ng) return 1;
     double dx = fabs((double)xp - xg[gi]) / Lg;
     double dy = fabs((double)yp - xg[gj]) / Lg;
     double dz = fabs((double)zp - xg[gk]) / Lg;
@@ -325,50 +334,52 @@ static void cic_one(int Ng, double box, double pmass, double Lg, const double *x
     if (gjj == Ng) gjj = 0;
     if (gkk == Ng) gkk = 0;
     double L3 = pow(Lg, 3);
-    /* index k + Ng*(j + i*Ng) */
+    double *vs[3] = {vx, vy, vz};
+    float vp[3] = {vxp, vyp, vzp};
 #define IDX(i, j, k) ((long)(k) + (long)Ng * ((long)(j) + (long)(i) * Ng))
-    if (gi >= 0 && gi < Ng) {
-        rho[IDX(gi, gj, gk)] += tx * ty * tz * pmass / L3;
-        rho[IDX(gi, gjj, gk)] += tx * dy * tz * pmass / L3;
-        rho[IDX(gi, gj, gkk)] += tx * ty * dz * pmass / L3;
-        rho[IDX(gi, gjj, gkk)] += tx * dy * dz * pmass / L3;
-        float *vs[3] = {vx, vy, vz};
-        float vp[3] = {vxp, vyp, vzp};
+    /* the gi planes, then the gii planes, in the reference's order */
+    {
+        long o1 = IDX(gi, gj, gk), o2 = IDX(gi, gjj, gk), o3 = IDX(gi, gj, gkk), o4 = IDX(gi, gjj, gkk);
+        rho[o1] += tx * ty * tz * pmass / L3;
+        rho[o2] += tx * dy * tz * pmass / L3;
+        rho[o3] += tx * ty * dz * pmass / L3;
+        rho[o4] += tx * dy * dz * pmass / L3;
         for (int c = 0; c < 3; ++c) {
-            vs[c][IDX(gi, gj, gk)] += tx * ty * tz * vp[c] / L3;
-            vs[c][IDX(gi, gjj, gk)] += tx * dy * tz * vp[c] / L3;
-            vs[c][IDX(gi, gj, gkk)] += tx * ty * dz * vp[c] / L3;
-            vs[c][IDX(gi, gjj, gkk)] += tx * dy * dz * vp[c] / L3;
+            vs[c][o1] = st32(vs[c][o1] + tx * ty * tz * vp[c] / L3, f32);
+            vs[c][o2] = st32(vs[c][o2] + tx * dy * tz * vp[c] / L3, f32);
+            vs[c][o3] = st32(vs[c][o3] + tx * ty * dz * vp[c] / L3, f32);
+            vs[c][o4] = st32(vs[c][o4] + tx * dy * dz * vp[c] / L3, f32);
         }
     }
-    if (gii >= 0 && gii < Ng) {
-        rho[IDX(gii, gjj, gk)] += dx * dy * tz * pmass / L3;
-        rho[IDX(gii, gj, gk)] += dx * ty * tz * pmass / L3;
-        rho[IDX(gii, gj, gkk)] += dx * ty * dz * pmass / L3;
-        rho[IDX(gii, gjj, gkk)] += dx * dy * dz * pmass / L3;
-        float *vs[3] = {vx, vy, vz};
-        float vp[3] = {vxp, vyp, vzp};
+    {
+        long o1 = IDX(gii, gjj, gk), o2 = IDX(gii, gj, gk), o3 = IDX(gii, gj, gkk), o4 = IDX(gii, gjj, gkk);
+        rho[o1] += dx * dy * tz * pmass / L3;
+        rho[o2] += dx * ty * tz * pmass / L3;
+        rho[o3] += dx * ty * dz * pmass / L3;
+        rho[o4] += dx * dy * dz * pmass / L3;
         for (int c = 0; c < 3; ++c) {
-            vs[c][IDX(gii, gjj, gk)] += dx * dy * tz * vp[c] / L3;
-            vs[c][IDX(gii, gj, gk)] += dx * ty * tz * vp[c] / L3;
-            vs[c][IDX(gii, gj, gkk)] += dx * ty * dz * vp[c] / L3;
-            vs[c][IDX(gii, gjj, gkk)] += dx * dy * dz * vp[c] / L3;
+            vs[c][o1] = st32(vs[c][o1] + dx * dy * tz * vp[c] / L3, f32);
+            vs[c][o2] = st32(vs[c][o2] + dx * ty * tz * vp[c] / L3, f32);
+            vs[c][o3] = st32(vs[c][o3] + dx * ty * dz * vp[c] / L3, f32);
+            vs[c][o4] = st32(vs[c][o4] + dx * dy * dz * vp[c] / L3, f32);
         }
     }
 #undef IDX
+    return 0;
 }
 
 /* Deposit one IC chunk of np particles (pos, vel: 3*np floats) the way
  * read_gadget does at one rank (src/gadget_IC.h:610-642).  ref_p1 != 0
  * also deposits the rank's "left ghost" copies: the trailing particles of
  * the x-sorted chunk with x > xg[N-1] (Pre_DistributeParticles_withGhost,
- * :445-448), which at one rank are then deposited twice.  The extra particle
+ * :445-448), which at one rank are thus deposited twice.  The extra particle
  * the reference reads past the end of the chunk (:431-432) is not
- * reproduced. */
-void oracle_cic_chunk(int Ng, double box, double pmass, const float *pos, const float *vel, long np,
-                      int ref_p1, double *rho, float *vx, float *vy, float *vz) {
+ * reproduced.  Returns the number of skipped particles. */
+long oracle_cic_chunk(int Ng, double box, double pmass, const float *pos, const float *vel, long np,
+                      int ref_p1, int f32, double *rho, double *vx, double *vy, double *vz) {
     double Lg = box / Ng;
     double *xg = (double *)malloc(sizeof(double) * Ng);
+    long skipped = 0;
     for (int i = 0; i < Ng; ++i) xg[i] = i * Lg + 0.5 * Lg;
     if (ref_p1) {
         long ng = 0;
@@ -377,30 +388,32 @@ void oracle_cic_chunk(int Ng, double box, double pmass, const float *pos, const 
             else break;
         }
         for (long q = np - ng; q < np; ++q)
-            cic_one(Ng, box, pmass, Lg, xg, pos[3 * q], pos[3 * q + 1], pos[3 * q + 2],
-                    vel[3 * q], vel[3 * q + 1], vel[3 * q + 2], rho, vx, vy, vz);
+            skipped += cic_one(Ng, box, pmass, Lg, xg, pos[3 * q], pos[3 * q + 1], pos[3 * q + 2],
+                               vel[3 * q], vel[3 * q + 1], vel[3 * q + 2], rho, vx, vy, vz, f32);
     }
     for (long q = 0; q < np; ++q)
-        cic_one(Ng, box, pmass, Lg, xg, pos[3 * q], pos[3 * q + 1], pos[3 * q + 2],
-                vel[3 * q], vel[3 * q + 1], vel[3 * q + 2], rho, vx, vy, vz);
+        skipped += cic_one(Ng, box, pmass, Lg, xg, pos[3 * q], pos[3 * q + 1], pos[3 * q + 2],
+                           vel[3 * q], vel[3 * q + 1], vel[3 * q + 2], rho, vx, vy, vz, f32);
     free(xg);
+    return skipped;
 }
 
 /* After all chunks: mass-weighted velocity (src/gadget_IC.h:645-651), the
  * velocity-divergence phase (initial_velocity_Poisson_Solver, :157-247) and
- * the wave function (:659-670).  psi (out) N^3 complex; theta (out, may be
+ * the wave function (:659-670).  psi (out) N^3 complex; theta_out (may be
  * NULL) N^3 complex, the phase field opr.phik.  a: scale factor of the IC
- * header; dx: cell size in Mpc (par.dx). */
+ * header; dx: cell size in Mpc (par.dx); h0, m2Mpc, s2Myr: constants of
+ * src/Cosmoinput3D.h. */
 void oracle_ic_assemble(const oracle_params *p, double pmass, double a, double dx, double h0,
-                        double m2Mpc_, double s2Myr_, double *rho, float *vx, float *vy, float *vz,
+                        double m2Mpc_, double s2Myr_, int f32, double *rho, double *vx, double *vy, double *vz,
                         double *psi, double *theta_out) {
     int N = p->n;
     long n3 = (long)N * N * N;
     for (long i = 0; i < n3; ++i) {
         if (rho[i] != 0) {
-            vx[i] *= pmass / rho[i];
-            vy[i] *= pmass / rho[i];
-            vz[i] *= pmass / rho[i];
+            vx[i] = st32(vx[i] * (pmass / rho[i]), f32);
+            vy[i] = st32(vy[i] * (pmass / rho[i]), f32);
+            vz[i] = st32(vz[i] * (pmass / rho[i]), f32);
         }
     }
     cd *th = (cd *)malloc(sizeof(cd) * (size_t)n3);
@@ -411,11 +424,12 @@ void oracle_ic_assemble(const oracle_params *p, double pmass, double a, double d
                 int ip = (i + 1) % N, im = (i + N - 1) % N;     /* ghost planes at one rank */
                 int jp = (j + 1) % N, jm = (j + N - 1) % N;
                 int kp = (k + 1) % N, km = (k + N - 1) % N;
-                float dvx = vx[k + (long)N * (j + (long)ip * N)] - vx[k + (long)N * (j + (long)im * N)];
-                float dvy = vy[k + (long)N * (jp + (long)i * N)] - vy[k + (long)N * (jm + (long)i * N)];
-                float dvz = vz[kp + (long)N * (j + (long)i * N)] - vz[km + (long)N * (j + (long)i * N)];
+                double dvx = st32(vx[k + (long)N * (j + (long)ip * N)] - vx[k + (long)N * (j + (long)im * N)], f32);
+                double dvy = st32(vy[k + (long)N * (jp + (long)i * N)] - vy[k + (long)N * (jm + (long)i * N)], f32);
+                double dvz = st32(vz[kp + (long)N * (j + (long)i * N)] - vz[km + (long)N * (j + (long)i * N)], f32);
+                double sum = st32(st32(dvx + dvy, f32) + dvz, f32);
                 long o = k + (long)N * (j + (long)i * N);
-                th[o].re = 0.5 * (dvx + dvy + dvz) * unit / dx;  /* :216 */
+                th[o].re = 0.5 * sum * unit / dx;                /* :216 */
                 th[o].im = 0.0;
             }
     fft_axis(th, N, N, N, 2, -1);
@@ -440,12 +454,12 @@ void oracle_ic_assemble(const oracle_params *p, double pmass, double a, double d
     double cph = p->mfdm / pow(p->c, 2) / p->hbar;
     cd *ps = (cd *)psi;
     for (long i = 0; i < n3; ++i) {
-        /* 1.225 sqrt(rho u / m_s) exp(c theta i), theta complex (:666) */
+        /* 1.225 sqrt(rho u / m_s) exp(c theta i) with complex theta (:666):
+         * exp(i c (re + i im)) = exp(-c im) (cos(c re) + i sin(c re)) */
         double amp = 1.225 * sqrt(rho[i] * uc / p->mfdm_solar);
-        double er = -cph * th[i].im, ei = cph * th[i].re;   /* exp(i c (re + i im)) = exp(-c im) e^{i c re} */
-        double m = exp(er);
-        ps[i].re = amp * m * cos(ei);
-        ps[i].im = amp * m * sin(ei);
+        double m = exp(-cph * th[i].im), ang = cph * th[i].re;
+        ps[i].re = amp * m * cos(ang);
+        ps[i].im = amp * m * sin(ang);
     }
     if (theta_out) memcpy(theta_out, th, sizeof(cd) * (size_t)n3);
     free(th);

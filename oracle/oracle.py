@@ -87,6 +87,12 @@ def lib():
         L.oracle_drift.argtypes = [pp, dp, ctypes.c_double]
         L.oracle_split_op.argtypes = [pp, dp, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.oracle_maxima.argtypes = [pp, dp, dp, ctypes.c_int, dp]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.oracle_cic_chunk.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, fp, fp, ctypes.c_long,
+                                       ctypes.c_int, ctypes.c_int, dp, dp, dp, dp]
+        L.oracle_cic_chunk.restype = ctypes.c_long
+        L.oracle_ic_assemble.argtypes = [pp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, dp, dp, dp, dp, dp]
         L.oracle_num_threads.restype = ctypes.c_int
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         _LIB = L
@@ -257,6 +263,36 @@ def simulate(rp: RunParams, psi0: np.ndarray, a0: float, nsteps: int, ref_maxphi
         used.append((dKE, dPE, dQI, a))
         st.split_op(dKE, dPE, dQI)                                     # :304
     return st, used
+
+
+def ic_from_particles(rp: RunParams, chunks, box: float, pmass: float, a: float, ref_p1: bool = False,
+                      f32: bool = False):
+    """read_gadget after the file reads (src/gadget_IC.h:610-672): CIC of
+    every chunk, mass-weighted velocities, velocity-divergence phase and
+    psi.  box kpc/h, pmass Msun/h.  f32 reproduces the reference's float
+    velocity grids.  Returns (psi, theta, skipped)."""
+    n = rp.n
+    rho = np.zeros((n, n, n))
+    vs = [np.zeros((n, n, n)) for _ in range(3)]
+    fp = ctypes.POINTER(ctypes.c_float)
+    skipped = 0
+    for pos, vel in chunks:
+        p = np.ascontiguousarray(pos, dtype=np.float32)
+        v = np.ascontiguousarray(vel, dtype=np.float32)
+        skipped += lib().oracle_cic_chunk(n, box, pmass, p.ctypes.data_as(fp), v.ctypes.data_as(fp), p.shape[0],
+                                          int(ref_p1), int(f32), _dp(rho), _dp(vs[0]), _dp(vs[1]), _dp(vs[2]))
+    psi = np.zeros((n, n, n), np.complex128)
+    theta = np.zeros((n, n, n), np.complex128)
+    cp = rp.c_params()
+    lib().oracle_ic_assemble(ctypes.byref(cp), pmass, a, rp.dx, h_0, m2Mpc, s2Myr, int(f32), _dp(rho),
+                             _dp(vs[0]), _dp(vs[1]), _dp(vs[2]), _dp(psi.view(np.float64)),
+                             _dp(theta.view(np.float64)))
+    return psi, theta, skipped
+
+
+def header_rhoc(nchunks: int, npart_chunk: int, pmass: float, box: float) -> float:
+    """par.rhoc from the Gadget header, src/gadget_IC.h:319 (pmass Msun/h)."""
+    return nchunks * npart_chunk * pmass / math.pow(box, 3) * math.pow(1e3, 3) * math.pow(h_0, 2)
 
 
 def set_threads(t: int):

@@ -1,0 +1,198 @@
+"""GPU: initial conditions, snapshots and the command-line driver against
+the oracle and the reference's file formats."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from tests.fields import rel_l2
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _mods():
+    import ifdm_code_amd as F
+    from ifdm_code_amd import icgen
+    from oracle import oracle as O
+    return F, icgen, O
+
+
+@pytest.mark.parametrize("n,ghosts", [(16, False), (32, False), (32, True)])
+def test_ic_matches_oracle(n, ghosts):
+    F, icgen, O = _mods()
+    _say(f"[ic] n={n} ghosts={ghosts}")
+    pos, vel, pm, a = icgen.zeldovich(n, 10.0, seed=n)
+    order = np.argsort(pos[:, 0], kind="stable")
+    pos, vel = pos[order], vel[order]
+    per = n ** 3 // 4
+    chunks = [(pos[c * per:(c + 1) * per], vel[c * per:(c + 1) * per]) for c in range(4)]
+    rp = O.RunParams(n=n, xmax_in=5.0)
+    psi_o, th_o, _ = O.ic_from_particles(rp, chunks, 10.0e3, pm * 1e10, a, ref_p1=ghosts)
+    cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n)
+    with F.Solver(cfg) as s:
+        th, skipped = s.ic_from_particles(chunks, 10.0e3, pm * 1e10, a, ref_ghosts=ghosts, want_theta=True)
+        psi = s.get_psi()
+    assert skipped == 0
+    assert rel_l2(th, th_o) < 1e-11
+    assert rel_l2(psi, psi_o) < 1e-12
+
+
+def test_ic_mixed_precision():
+    F, icgen, O = _mods()
+    n = 32
+    _say(f"[ic mixed] n={n}")
+    pos, vel, pm, a = icgen.zeldovich(n, 10.0, seed=2)
+    rp = O.RunParams(n=n)
+    psi_o, _, _ = O.ic_from_particles(rp, [(pos, vel)], 10.0e3, pm * 1e10, a)
+    cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n, precision=F.MIXED)
+    with F.Solver(cfg) as s:
+        s.ic_from_particles([(pos, vel)], 10.0e3, pm * 1e10, a)
+        psi = s.get_psi()
+    assert rel_l2(psi, psi_o) < 1e-7
+
+
+def test_read_gadget(tmp_path):
+    F, icgen, O = _mods()
+    n = 16
+    _say(f"[read_gadget] n={n}")
+    pos, vel, pm, a = icgen.zeldovich(n, 10.0, seed=4)
+    chunks = icgen.write_gadget(str(tmp_path / "ic" / "sorted"), pos, vel, pm, a, 10.0e3)
+    rp = O.RunParams(n=n)
+    psi_o, _, _ = O.ic_from_particles(rp, chunks, 10.0e3, pm * 1e10, a)
+    cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n)
+    vout = str(tmp_path / "voutput.dat")
+    with F.Solver(cfg) as s:
+        info = s.read_gadget(str(tmp_path / "ic" / "sorted"), 4, voutput=vout)
+        psi = s.get_psi()
+    assert info["npart"] == n ** 3 and info["skipped"] == 0 and info["a"] == pytest.approx(a)
+    assert info["rhoc"] == pytest.approx(O.header_rhoc(4, n ** 3 // 4, pm * 1e10, 10.0e3), rel=1e-14)
+    assert rel_l2(psi, psi_o) < 1e-12
+    assert os.path.getsize(vout) == 4 * n ** 3
+
+
+def test_read_gadget_errors(tmp_path):
+    F, icgen, O = _mods()
+    n = 8
+    pos, vel, pm, a = icgen.zeldovich(n, 10.0, seed=4)
+    icgen.write_gadget(str(tmp_path / "ic"), pos, vel, pm, a, 10.0e3)
+    # box mismatch: the IC is 10 Mpc/h, the run asks for xmax 4 (8 Mpc/h)
+    with F.Solver(F.Config.from_cli(2.5e-24, 0.0, 4.0, n)) as s:
+        with pytest.raises(F.IfdmError, match="Boxsize"):
+            s.read_gadget(str(tmp_path / "ic"), 4)
+    with F.Solver(F.Config.from_cli(2.5e-24, 0.0, 5.0, n)) as s:
+        with pytest.raises(F.IfdmError, match="cannot read IC file"):
+            s.read_gadget(str(tmp_path / "missing"), 4)
+        # a truncated chunk
+        with open(tmp_path / "ic.3", "r+b") as f:
+            f.truncate(1000)
+        with pytest.raises(F.IfdmError, match="truncated"):
+            s.read_gadget(str(tmp_path / "ic"), 4)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_snapshot_layout_and_rerun(tmp_path, prec):
+    """output<t>.dat = [mfdm][g][a][x[N]][psi N^3 complex128] (write3D.h),
+    read back by the rerun path; a restart continues bit-identically."""
+    F, icgen, O = _mods()
+    n = 16
+    _say(f"[snapshot] prec={prec}")
+    pos, vel, pm, a = icgen.zeldovich(n, 10.0, seed=6)
+    cfg = F.Config.from_cli(2.5e-24, 1e-60, 5.0, n, precision=prec)
+    path = str(tmp_path / "output7.dat")
+    with F.Solver(cfg) as s:
+        s.ic_from_particles([(pos, vel)], 10.0e3, pm * 1e10, a)
+        a1, sched = s.simulate_native(a, 3)
+        s.write_snapshot(path, a1, 2.5e-24, 1e-60)
+        psi = s.get_psi()
+        a2, _ = s.simulate_native(a1, 3)
+        cont = s.get_psi()
+    raw = np.fromfile(path, np.float64)
+    assert raw.size == 3 + n + 2 * n ** 3
+    assert raw[0] == 2.5e-24 and raw[1] == 1e-60 and raw[2] == a1
+    xmax = cfg.xmax
+    assert np.allclose(raw[3:3 + n], xmax / n - xmax + np.arange(n) * (2 * xmax / n), rtol=0, atol=1e-15)
+    assert np.array_equal(raw[3 + n:].view(np.complex128).reshape(n, n, n), psi)
+    with F.Solver(cfg) as s2:
+        ar, m, g = s2.read_snapshot(path)
+        assert (ar, m, g) == (a1, 2.5e-24, 1e-60)
+        assert np.array_equal(s2.get_psi(), psi) if prec == 0 else rel_l2(s2.get_psi(), psi) < 1e-7
+        a3, _ = s2.simulate_native(ar, 3)
+        again = s2.get_psi()
+    assert a3 == a2
+    if prec == 0:
+        assert np.array_equal(again, cont)
+
+
+def test_readsnap_reads_our_snapshot(tmp_path):
+    """The reference's own reader (src/readsnap.cpp, built into oracle/_ref)
+    turns a library-written snapshot into the expected ASCII z-slice."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "readsnap")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/readsnap not built (needs the reference sources at build time)")
+    F, icgen, O = _mods()
+    n = 8
+    pos, vel, pm, a = icgen.zeldovich(n, 10.0, seed=8)
+    cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n)
+    d = str(tmp_path) + "/"
+    with F.Solver(cfg) as s:
+        s.ic_from_particles([(pos, vel)], 10.0e3, pm * 1e10, a)
+        s.write_snapshot(d + "output5.dat", a, 2.5e-24, 0.0)
+        psi = s.get_psi()
+    r = subprocess.run([exe, str(n), "3", "5", d, d], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    rows = np.loadtxt(d + "output5.dat.z3")
+    assert rows.shape == (n * n, 4)
+    ms = 2.5e-24 * O.kg2eV / O.kg2solar
+    x = cfg.xmax / n - cfg.xmax + np.arange(n) * (2 * cfg.xmax / n)
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    expect = np.stack([x[i].ravel(), x[j].ravel(), (np.abs(psi[:, :, 3]) ** 2 * ms).ravel(),
+                       np.angle(psi[:, :, 3]).ravel()], axis=1)
+    assert np.allclose(rows, expect, rtol=2e-5, atol=1e-5 * np.abs(expect).max(axis=0))
+
+
+def test_cli_matches_library_and_restarts(tmp_path):
+    """The 10-parameter driver on a Gadget IC, then a rerun from one of its
+    snapshots, against the library driven step by step from Python."""
+    F, icgen, O = _mods()
+    n = 16
+    _say(f"[cli] n={n}")
+    exe = os.path.join(ROOT, "ifdm_code_amd", "ifdm")
+    pos, vel, pm, a0 = icgen.zeldovich(n, 10.0, seed=12)
+    icgen.write_gadget(str(tmp_path / "ic" / "sorted"), pos, vel, pm, a0, 10.0e3)
+    out = str(tmp_path / "out") + "/"
+    os.makedirs(out)
+    args = [exe, "2.5e-24", "0", "5.", str(n), "3", "7", "0", out, str(tmp_path / "ic" / "sorted"), "0",
+            "--no-a-triggers"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    snaps = sorted(os.listdir(out))
+    assert "output1.dat" in snaps and "output4.dat" in snaps and "output7.dat" in snaps and "voutput.dat" in snaps
+    # the same run through the Python binding: snapshot t holds psi before step t-1
+    cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n)
+    with F.Solver(cfg) as s:
+        info = s.read_gadget(str(tmp_path / "ic" / "sorted"), 4)
+        st = s.poisson()
+        a = info["a"]
+        trig = F.OutputTriggers(0, False)
+        for tn in range(7):
+            dt, a, _ = F.timestep_criteria(cfg, a, st)
+            t, _stop = trig(a, tn, 3, 7)
+            if t is not None:
+                raw = np.fromfile(out + f"output{t}.dat", np.float64)
+                assert raw[2] == a
+                assert rel_l2(raw[3 + n:].view(np.complex128).reshape(n, n, n), s.get_psi()) < 1e-13
+            st = s.step(*dt)
+    # rerun from output4.dat (ti = 4) for the remaining steps
+    out2 = str(tmp_path / "out2") + "/"
+    os.makedirs(out2)
+    args = [exe, "2.5e-24", "0", "5.", str(n), "3", "7", "4", out2, out, "1", "--no-a-triggers"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert os.path.exists(out2 + "output7.dat")
