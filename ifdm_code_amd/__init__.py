@@ -76,6 +76,7 @@ class _ICInfo(ctypes.Structure):
 
 
 EXPORTS = [
+    "ifdm_run_checkpointed", "ifdm_adjoint", "ifdm_adjoint_device", "ifdm_set_psi_device", "ifdm_get_psi_device",
     "ifdm_read_gadget", "ifdm_ic_begin", "ifdm_ic_deposit", "ifdm_ic_finish", "ifdm_set_rhoc",
     "ifdm_write_snapshot", "ifdm_read_snapshot",
     "ifdm_create", "ifdm_destroy", "ifdm_last_error", "ifdm_create_error", "ifdm_set_psi", "ifdm_get_psi",
@@ -112,6 +113,11 @@ def lib():
     L.ifdm_simulate.argtypes = [vp, ctypes.c_double, ip, dp, ctypes.POINTER(_Dt)]
     L.ifdm_time_passes.argtypes = [vp, ctypes.POINTER(_Dt), ip, ctypes.POINTER(ctypes.c_float)]
     L.ifdm_fft3d.argtypes = [vp, ip]
+    L.ifdm_run_checkpointed.argtypes = [vp, ctypes.POINTER(_Dt), ip, ip]
+    L.ifdm_adjoint.argtypes = [vp, ctypes.POINTER(_Dt), ip, dp]
+    L.ifdm_adjoint_device.argtypes = [vp, ctypes.POINTER(_Dt), ip, vp]
+    L.ifdm_set_psi_device.argtypes = [vp, vp]
+    L.ifdm_get_psi_device.argtypes = [vp, vp]
     fp = ctypes.POINTER(ctypes.c_float)
     ll = ctypes.c_longlong
     L.ifdm_read_gadget.argtypes = [vp, ctypes.c_char_p, ip, ip, ctypes.c_char_p, ctypes.POINTER(_ICInfo)]
@@ -252,6 +258,34 @@ class Solver:
         if a.size != self.cfg.n ** 3:
             raise ValueError("phi size mismatch")
         self._chk(lib().ifdm_set_phi(self._h, _dp(a)), "ifdm_set_phi")
+
+    # ---- adjoint
+    @staticmethod
+    def _sched(schedule):
+        m = len(schedule)
+        return (_Dt * max(m, 1))(*[_Dt(*x) for x in schedule]), m
+
+    def run_checkpointed(self, schedule, K: int):
+        arr, m = self._sched(schedule)
+        self._chk(lib().ifdm_run_checkpointed(self._h, arr, m, int(K)), "ifdm_run_checkpointed")
+
+    def adjoint(self, schedule, g_T: np.ndarray) -> np.ndarray:
+        """Gradient w.r.t. psi_0 from the gradient w.r.t. psi_T (both in the
+        dL/dRe + i dL/dIm convention) after run_checkpointed(schedule)."""
+        arr, m = self._sched(schedule)
+        g = np.ascontiguousarray(g_T, dtype=np.complex128).copy()
+        self._chk(lib().ifdm_adjoint(self._h, arr, m, _dp(g.view(np.float64))), "ifdm_adjoint")
+        return g
+
+    def adjoint_device(self, schedule, g_ptr: int):
+        arr, m = self._sched(schedule)
+        self._chk(lib().ifdm_adjoint_device(self._h, arr, m, ctypes.c_void_p(g_ptr)), "ifdm_adjoint_device")
+
+    def set_psi_device(self, ptr: int):
+        self._chk(lib().ifdm_set_psi_device(self._h, ctypes.c_void_p(ptr)), "ifdm_set_psi_device")
+
+    def get_psi_device(self, ptr: int):
+        self._chk(lib().ifdm_get_psi_device(self._h, ctypes.c_void_p(ptr)), "ifdm_get_psi_device")
 
     # ---- initial conditions and snapshots
     def read_gadget(self, icpath: str, nchunks: int = 4, ref_ghosts: bool = False, voutput: str | None = None):

@@ -1,0 +1,39 @@
+"""torch.autograd binding of the KDK evolution and its hand-written adjoint.
+
+``evolve(psi0, solver, schedule, K)`` returns psi after len(schedule) steps
+(frozen time steps) computed by libifdm.so; its backward is the library's
+reverse sweep (ifdm_adjoint_device) with checkpoints every K steps.  psi0 is
+a complex128 CUDA tensor on the solver's device, shape (N, N, N).  Gradients
+follow torch's convention for real losses of complex tensors,
+dL/dRe + i dL/dIm, which is the convention of ifdm_adjoint.
+"""
+from __future__ import annotations
+
+import torch
+
+
+class KDKEvolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, psi0, solver, schedule, K):
+        if psi0.dtype != torch.complex128 or not psi0.is_cuda:
+            raise TypeError("psi0 must be a complex128 CUDA tensor")
+        src = psi0.detach().contiguous()
+        torch.cuda.synchronize(src.device)
+        solver.set_psi_device(src.data_ptr())
+        solver.run_checkpointed(schedule, int(K))
+        out = torch.empty_like(src)
+        solver.get_psi_device(out.data_ptr())
+        ctx.solver = solver
+        ctx.schedule = list(schedule)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        g = g.detach().contiguous().clone()
+        torch.cuda.synchronize(g.device)
+        ctx.solver.adjoint_device(ctx.schedule, g.data_ptr())
+        return g, None, None, None
+
+
+def evolve(psi0: torch.Tensor, solver, schedule, K: int = 20) -> torch.Tensor:
+    return KDKEvolve.apply(psi0, solver, schedule, K)

@@ -46,9 +46,18 @@ struct ifdm_ctx {
     unsigned long long *ic_skipped = nullptr;
     double ic_box = 0.0, ic_pmass = 0.0;
     long long ic_npart = 0;
+    // adjoint: gradient field, the real source s and its Poisson image,
+    // and the forward checkpoints (psi every ck_k steps, on the device)
+    void *adj_g = nullptr, *adj_s = nullptr, *adj_phs = nullptr;
+    void *ck = nullptr;
+    int ck_k = 0, ck_count = 0, ck_nsteps = -1;
     // launch schedule: 0 = one launch per pass; S > 0 = the x-slab-local
     // chains (A,B), (D,E,F), (H,I) run slab by slab, S x-planes per slab
     int slab = 0;
+    // strided passes: 0 = one tile per work-group; 1 = persistent
+    // work-groups with next-tile prefetch
+    int persistent = 0;
+    int num_cu = 0;
     // optional live per-pass timing (ifdm_set_profiling)
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;          // 2 events per profiled launch

@@ -26,7 +26,7 @@ __device__ __forceinline__ double xg_of(int i, double lg) { return i * lg + 0.5 
 
 // One thread per particle.  Returns nothing; particles outside
 // [-lg/2, box) in some coordinate are counted in *skipped and not deposited.
-__global__ __launch_bounds__(256) void k_cic(const float *__restrict__ pos, const float *__restrict__ vel, long np,
+static __global__ __launch_bounds__(256) void k_cic(const float *__restrict__ pos, const float *__restrict__ vel, long np,
                                              CicArgs a, double *__restrict__ rho, double *__restrict__ vx,
                                              double *__restrict__ vy, double *__restrict__ vz,
                                              unsigned long long *__restrict__ skipped) {
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void k_cic(const float *__restrict__ pos, cons
 }
 
 // v *= pmass / rho where rho != 0 (src/gadget_IC.h:645-651)
-__global__ __launch_bounds__(256) void k_ic_velnorm(const double *__restrict__ rho, double *__restrict__ vx,
+static __global__ __launch_bounds__(256) void k_ic_velnorm(const double *__restrict__ rho, double *__restrict__ vx,
                                                     double *__restrict__ vy, double *__restrict__ vz, long n3,
                                                     double pmass) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_ic_velnorm(const double *__restrict__ r
 
 // theta source: 0.5 (dvx + dvy + dvz) * unit / dx with periodic central
 // differences (src/gadget_IC.h:190-219 at one rank), as a complex field
-__global__ __launch_bounds__(256) void k_ic_divergence(const double *__restrict__ vx, const double *__restrict__ vy,
+static __global__ __launch_bounds__(256) void k_ic_divergence(const double *__restrict__ vx, const double *__restrict__ vy,
                                                        const double *__restrict__ vz, cx<double> *__restrict__ out,
                                                        int n, double scale) {
     const long n3 = (long)n * n * n;
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_ic_divergence(const double *__restrict_
 
 // psi = 1.225 sqrt(rho uc / m_s) exp(i cph theta), theta complex
 // (src/gadget_IC.h:666); psi overwrites theta in place
-__global__ __launch_bounds__(256) void k_ic_assemble(const double *__restrict__ rho, cx<double> *__restrict__ th,
+static __global__ __launch_bounds__(256) void k_ic_assemble(const double *__restrict__ rho, cx<double> *__restrict__ th,
                                                      long n3, double uc, double ms, double cph) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
         const cx<double> t = th[i];
@@ -125,6 +125,15 @@ __global__ __launch_bounds__(256) void k_store_scaled(const cx<double> *__restri
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
         const cx<double> z = src[i];
         dst[i] = mkc<T>((T)ldexp(z.x, -scale_exp), (T)ldexp(z.y, -scale_exp));
+    }
+}
+
+// storage precision -> FP64 field with the inverse power-of-two scale
+static __global__ __launch_bounds__(256) void k_load_scaled(const cx<float> *__restrict__ src, cx<double> *__restrict__ dst,
+                                                     long n3, int scale_exp) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
+        const cx<float> z = src[i];
+        dst[i] = mkc<double>(ldexp((double)z.x, scale_exp), ldexp((double)z.y, scale_exp));
     }
 }
 

@@ -14,6 +14,7 @@
 #include "constants.h"
 #include "ctx.h"
 #include "kdk_passes.h"
+#include "ic_kernels.h"
 
 using namespace ifdm;
 
@@ -71,11 +72,26 @@ struct Ops {
         constexpr int TC = s_cols<T>();
         if (no == 0) no = N;
         sa.other0 = o0;
-        dim3 grid((kv + TC - 1) / TC, no);
+        const int ntz = (kv + TC - 1) / TC;
         int rc;
         if ((rc = prof_begin(c))) return rc;
-        hipLaunchKernelGGL((k_spass<T, N, AXIS, MODE>), grid, dim3(TC * P), 0, c->stream,
-                           (cx<T> *)arr, rl, kv, (const cx<T> *)c->tw, sa);
+        if (c->persistent) {
+            static int occ = -1;   // resident work-groups per CU for this instantiation
+            if (occ < 0) {
+                int o = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_spass_pp<T, N, AXIS, MODE>, TC * P, 0) !=
+                        hipSuccess || o < 1)
+                    o = 1;
+                occ = o;
+            }
+            const int ntiles = ntz * no;
+            const int nb = std::min(ntiles, occ * c->num_cu);
+            hipLaunchKernelGGL((k_spass_pp<T, N, AXIS, MODE>), dim3(nb), dim3(TC * P), 0, c->stream,
+                               (cx<T> *)arr, rl, kv, (const cx<T> *)c->tw, sa, ntz, ntiles);
+        } else {
+            hipLaunchKernelGGL((k_spass<T, N, AXIS, MODE>), dim3(ntz, no), dim3(TC * P), 0, c->stream,
+                               (cx<T> *)arr, rl, kv, (const cx<T> *)c->tw, sa);
+        }
         LAUNCHCHK(c);
         return prof_end(c);
     }
@@ -114,6 +130,32 @@ struct Ops {
             }
             default: return IFDM_E_ARG;
         }
+    }
+
+    // phi_dst = G(m_s r) for a real field r (k = 0 -> 0): the adjoint's Poisson solve
+    static int poisson_real(ifdm_ctx *c, void *src, void *dst) {
+        void *save = c->phi;
+        c->phi = src;
+        int rc = zpass<Z_RSRC>(c, zargs(c, 0, 0), nullptr);
+        c->phi = dst;
+        for (int i = 5; i <= 8 && !rc; ++i) rc = pass(c, i, 0, 0, 0);
+        c->phi = save;
+        return rc;
+    }
+
+    // reverse of one half-kick (aphi, arho) on (psi, adj_g) with the current phi
+    static int adj_kick(ifdm_ctx *c, double aphi, double arho) {
+        const unsigned nb = (unsigned)std::min<long>((c->n3 + 255) / 256, 65536);
+        hipLaunchKernelGGL((k_adj_src<T>), dim3(nb), dim3(256), 0, c->stream, (const cx<T> *)c->psi,
+                           (const cx<T> *)c->adj_g, (T *)c->adj_s, c->n3);
+        LAUNCHCHK(c);
+        int rc;
+        if ((rc = poisson_real(c, c->adj_s, c->adj_phs))) return rc;
+        hipLaunchKernelGGL((k_adj_kick<T>), dim3(nb), dim3(256), 0, c->stream, (cx<T> *)c->psi, (cx<T> *)c->adj_g,
+                           (const T *)c->phi, (const T *)c->adj_phs, (const T *)c->adj_s, c->n3, aphi,
+                           arho * c->psc2);
+        LAUNCHCHK(c);
+        return 0;
     }
 
     static int poisson(ifdm_ctx *c) {
@@ -224,6 +266,7 @@ struct Ops {
 };
 
 struct Dispatch {
+    int (*adj_kick)(ifdm_ctx *, double, double);
     int (*theta_solve)(ifdm_ctx *, double);
     int (*pass)(ifdm_ctx *, int, double, double, double);
     int (*poisson)(ifdm_ctx *);
@@ -235,7 +278,7 @@ struct Dispatch {
 
 template <typename T, int N>
 Dispatch make_dispatch() {
-    return Dispatch{&Ops<T, N>::theta_solve, &Ops<T, N>::pass, &Ops<T, N>::poisson, &Ops<T, N>::kick_drift_poisson, &Ops<T, N>::drift, &Ops<T, N>::fft3d,
+    return Dispatch{&Ops<T, N>::adj_kick, &Ops<T, N>::theta_solve, &Ops<T, N>::pass, &Ops<T, N>::poisson, &Ops<T, N>::kick_drift_poisson, &Ops<T, N>::drift, &Ops<T, N>::fft3d,
                     &Ops<T, N>::kick};
 }
 
@@ -312,7 +355,8 @@ void release(ifdm_ctx *c) {
     void *icb[] = {c->ic_rho, c->ic_vx, c->ic_vy, c->ic_vz, c->ic_skipped};
     for (void *b : icb)
         if (b) hipFree(b);
-    void *bufs[] = {c->psi, c->phi, c->shat, c->tw, c->k2, c->part_rho, c->part_phi, c->red, c->red_stage, c->dtab};
+    void *bufs[] = {c->psi, c->phi, c->shat, c->tw, c->k2, c->part_rho, c->part_phi, c->red, c->red_stage, c->dtab,
+                    c->adj_g, c->adj_s, c->adj_phs, c->ck};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (c->red_host) hipHostFree(c->red_host);
@@ -370,6 +414,8 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     };
     hipError_t e = hipSetDevice(cfg->device);
     if (e != hipSuccess) return fail(IFDM_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    if ((e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, cfg->device)) != hipSuccess)
+        return fail(IFDM_E_HIP, std::string("hipDeviceGetAttribute: ") + hipGetErrorString(e));
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
         return fail(IFDM_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     // element sizes of the stored fields (reductions stay FP64)
@@ -656,6 +702,144 @@ int ifdm_internal_theta_solve(ifdm_ctx *c, void *buf64, double coef) {
     return rc;
 }
 
+// ---------------------------------------------------------------- adjoint
+int ifdm_run_checkpointed(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, int K) {
+    if (!c || !sched || nsteps < 0 || K <= 0) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    int rc;
+    if ((rc = ifdm_internal_flush(c))) return rc;
+    const size_t bytes = (c->mixed ? 2 * sizeof(float) : 2 * sizeof(double)) * (size_t)c->n3;
+    if (c->ck) {
+        hipFree(c->ck);
+        c->ck = nullptr;
+    }
+    c->ck_nsteps = -1;
+    const int nck = nsteps / K + 1;
+    if (hipMalloc(&c->ck, bytes * nck) != hipSuccess) {
+        c->ck = nullptr;
+        c->err = "hipMalloc failed for the adjoint checkpoints";
+        return IFDM_E_NOMEM;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->ck, c->psi, bytes, hipMemcpyDeviceToDevice, c->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        if ((rc = ifdm_step(c, &sched[s], nullptr))) return rc;
+        if ((s + 1) % K == 0) {
+            if ((rc = ifdm_internal_flush(c))) return rc;
+            HIPCHK(c, hipMemcpyAsync((char *)c->ck + bytes * ((s + 1) / K), c->psi, bytes, hipMemcpyDeviceToDevice,
+                                     c->stream));
+        }
+    }
+    if ((rc = ifdm_internal_flush(c))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ck_k = K;
+    c->ck_count = nck;
+    c->ck_nsteps = nsteps;
+    return IFDM_OK;
+}
+
+// reverse sweep on the device gradient adj_g (see ifdm.h)
+static int adjoint_sweep(ifdm_ctx *c, const ifdm_dt *sched, int nsteps) {
+    Dispatch d;
+    get_dispatch(c->n, c->cfg.precision, &d);
+    const size_t bytes = (c->mixed ? 2 * sizeof(float) : 2 * sizeof(double)) * (size_t)c->n3;
+    int rc;
+    if ((rc = d.poisson(c))) return rc;                      // phi of the final state
+    for (int n = nsteps - 1; n >= 0; --n) {
+        const double aphi = 0.5 * sched[n].dPE * c->m_over_hbar, arho = 0.5 * sched[n].dQI * c->g_over_hbar;
+        if ((rc = d.adj_kick(c, aphi, arho))) return rc;    // second half-kick
+        if ((rc = d.drift(c, -sched[n].dKE))) return rc;    // psi back through the drift
+        std::swap(c->psi, c->adj_g);
+        rc = d.drift(c, -sched[n].dKE);                      // g <- D^H g
+        std::swap(c->psi, c->adj_g);
+        if (rc) return rc;
+        if ((rc = d.poisson(c))) return rc;                  // phi at the start of step n
+        if ((rc = d.adj_kick(c, aphi, arho))) return rc;    // first half-kick
+        if (n % c->ck_k == 0) {                              // replace the reverse-evolved psi
+            HIPCHK(c, hipMemcpyAsync(c->psi, (char *)c->ck + bytes * (n / c->ck_k), bytes, hipMemcpyDeviceToDevice,
+                                     c->stream));
+            if ((rc = d.poisson(c))) return rc;
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
+}
+
+static int adjoint_prepare(ifdm_ctx *c, int nsteps) {
+    if (c->mixed) { c->err = "the adjoint runs in FP64 contexts"; return IFDM_E_STATE; }
+    if (!c->ck || c->ck_nsteps != nsteps) {
+        c->err = "ifdm_adjoint needs the checkpoints of an ifdm_run_checkpointed over the same steps";
+        return IFDM_E_STATE;
+    }
+    const size_t cs = 2 * sizeof(double) * (size_t)c->n3, rs = sizeof(double) * (size_t)c->n3;
+    if (!c->adj_g) {
+        if (hipMalloc(&c->adj_g, cs) != hipSuccess || hipMalloc(&c->adj_s, rs) != hipSuccess ||
+            hipMalloc(&c->adj_phs, rs) != hipSuccess) {
+            c->err = "hipMalloc failed for the adjoint fields";
+            return IFDM_E_NOMEM;
+        }
+    }
+    return ifdm_internal_flush(c);
+}
+
+int ifdm_adjoint(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, double *g) {
+    if (!c || !sched || !g || nsteps < 0) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    int rc;
+    if ((rc = adjoint_prepare(c, nsteps))) return rc;
+    const size_t cs = 2 * sizeof(double) * (size_t)c->n3;
+    HIPCHK(c, hipMemcpy(c->adj_g, g, cs, hipMemcpyHostToDevice));
+    if ((rc = adjoint_sweep(c, sched, nsteps))) return rc;
+    HIPCHK(c, hipMemcpy(g, c->adj_g, cs, hipMemcpyDeviceToHost));
+    return IFDM_OK;
+}
+
+int ifdm_adjoint_device(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, void *g_dev) {
+    if (!c || !sched || !g_dev || nsteps < 0) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    int rc;
+    if ((rc = adjoint_prepare(c, nsteps))) return rc;
+    const size_t cs = 2 * sizeof(double) * (size_t)c->n3;
+    HIPCHK(c, hipMemcpyAsync(c->adj_g, g_dev, cs, hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = adjoint_sweep(c, sched, nsteps))) return rc;
+    HIPCHK(c, hipMemcpyAsync(g_dev, c->adj_g, cs, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
+}
+
+int ifdm_set_psi_device(ifdm_ctx *c, const void *psi_dev) {
+    if (!c || !psi_dev) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    c->pending = false;
+    if (!c->mixed) {
+        HIPCHK(c, hipMemcpyAsync(c->psi, psi_dev, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToDevice,
+                                 c->stream));
+    } else {
+        const unsigned nb = (unsigned)std::min<long>((c->n3 + 255) / 256, 65536);
+        hipLaunchKernelGGL((k_store_scaled<float>), dim3(nb), dim3(256), 0, c->stream, (const cx<double> *)psi_dev,
+                           (cx<float> *)c->psi, c->n3, c->psi_scale);
+        LAUNCHCHK(c);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
+}
+
+int ifdm_get_psi_device(ifdm_ctx *c, void *psi_dev) {
+    if (!c || !psi_dev) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (int rc0 = ifdm_internal_flush(c)) return rc0;
+    if (!c->mixed) {
+        HIPCHK(c, hipMemcpyAsync(psi_dev, c->psi, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToDevice,
+                                 c->stream));
+    } else {
+        const unsigned nb = (unsigned)std::min<long>((c->n3 + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_load_scaled, dim3(nb), dim3(256), 0, c->stream, (const cx<float> *)c->psi,
+                           (cx<double> *)psi_dev, c->n3, c->psi_scale);
+        LAUNCHCHK(c);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
+}
+
 int ifdm_stream_copy(ifdm_ctx *c, size_t bytes, int iters, double *gbs) {
     if (!c || !gbs || iters <= 0 || bytes < 16) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
@@ -697,6 +881,11 @@ int ifdm_set_option(ifdm_ctx *c, const char *name, long value) {
     if (std::strcmp(name, "slab_planes") == 0) {
         if (value < 0 || value > c->n) { c->err = "slab_planes out of range"; return IFDM_E_ARG; }
         c->slab = (int)value;
+        return IFDM_OK;
+    }
+    if (std::strcmp(name, "persistent") == 0) {
+        if (value < 0 || value > 1) { c->err = "persistent must be 0 or 1"; return IFDM_E_ARG; }
+        c->persistent = (int)value;
         return IFDM_OK;
     }
     c->err = std::string("unknown option ") + name;

@@ -53,7 +53,7 @@ struct SArgs {
     double fftnorm;    // 1 / N^3
 };
 
-enum ZMode { Z_FWD = 0, Z_INV = 1, Z_KICK_FWD = 2, Z_INV_SRC = 3, Z_SRC = 4, Z_C2R = 5 };
+enum ZMode { Z_FWD = 0, Z_INV = 1, Z_KICK_FWD = 2, Z_INV_SRC = 3, Z_SRC = 4, Z_C2R = 5, Z_RSRC = 6 };
 enum SMode { S_FWD = 0, S_INV = 1, S_DRIFT = 2, S_GREEN = 3 };
 
 template <int N> constexpr int z_lines() {
@@ -138,6 +138,22 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         const double bm = block_reduce<true>(mx, sb, tid, NT);
         if (tid == 0) part[blk] = bm;
         return;
+    } else if constexpr (MODE == Z_RSRC) {
+        // Poisson source from a real field r (read through `phi`): ms * r
+        const T *rrow = phi + line * N;
+        cx<T> s[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) s[e] = mkc<T>(valid ? (T)(za.ms * (double)rrow[t + e * P]) : (T)0, (T)0);
+        fft_line<T, N, -1>(s, L, t, tw);
+        if (valid) {
+            cx<T> *srow = shat + line * HS;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int p = t + e * P;
+                if (p <= N / 2) srow[p] = s[e];
+            }
+        }
+        return;
     } else {
         cx<T> *row = psi + line * N;
 #pragma unroll
@@ -198,23 +214,12 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
 // AXIS 1: lines along y (stride rl); AXIS 0: lines along x (stride N*rl).
 // A work-group takes TC consecutive z-columns so each global access of a
 // wavefront covers 128-byte row segments.
-template <typename T, int N, int AXIS, int MODE>
-__global__ __launch_bounds__(s_cols<T>() * (N / Sched<N>::E))
-void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw, SArgs sa) {
-    constexpr int E = Sched<N>::E, P = N / E, TC = s_cols<T>(), LS = lds_line<N>();
-    __shared__ cx<T> lds[TC * LS];
-    const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
-    const int k = blockIdx.x * TC + c;
-    const int other = sa.other0 + (int)blockIdx.y;
-    const bool valid = k < kv;
-    long base, stride;
-    if constexpr (AXIS == 1) { base = (long)other * N * rl + k; stride = rl; }
-    else { base = (long)other * rl + k; stride = (long)N * rl; }
-    cx<T> *L = lds + c * LS;
-    cx<T> v[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = valid ? a[base + (long)(t + e * P) * stride] : mkc<T>(0, 0);
 
+// the transform (and the spectral multiply of passes C and G) of one tile
+template <typename T, int N, int MODE>
+__device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const cx<T> *__restrict__ tw,
+                                              const SArgs &sa, int other, int k, bool valid) {
+    constexpr int E = Sched<N>::E, P = N / E;
     if constexpr (MODE == S_FWD) fft_line<T, N, -1>(v, L, t, tw);
     else if constexpr (MODE == S_INV) fft_line<T, N, +1>(v, L, t, tw);
     else {
@@ -236,9 +241,87 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
         }
         fft_line<T, N, +1>(v, L, t, tw);
     }
+}
+
+template <int N, int AXIS>
+__device__ __forceinline__ void spass_addr(int other, int k, int rl, long &base, long &stride) {
+    if constexpr (AXIS == 1) { base = (long)other * N * rl + k; stride = rl; }
+    else { base = (long)other * rl + k; stride = (long)N * rl; }
+}
+
+// one tile per work-group: grid (z-column tiles, lines of the fixed axis)
+template <typename T, int N, int AXIS, int MODE>
+__global__ __launch_bounds__(s_cols<T>() * (N / Sched<N>::E))
+void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw, SArgs sa) {
+    constexpr int E = Sched<N>::E, P = N / E, TC = s_cols<T>(), LS = lds_line<N>();
+    __shared__ cx<T> lds[TC * LS];
+    const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
+    const int k = blockIdx.x * TC + c;
+    const int other = sa.other0 + (int)blockIdx.y;
+    const bool valid = k < kv;
+    long base, stride;
+    spass_addr<N, AXIS>(other, k, rl, base, stride);
+    cx<T> *L = lds + c * LS;
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = valid ? a[base + (long)(t + e * P) * stride] : mkc<T>(0, 0);
+    spass_compute<T, N, MODE>(v, L, t, tw, sa, other, k, valid);
     if (valid) {
 #pragma unroll
         for (int e = 0; e < E; ++e) a[base + (long)(t + e * P) * stride] = v[e];
+    }
+}
+
+// Persistent variant: each work-group walks tiles blockIdx.x, +gridDim.x, ...
+// (tile = local line of the fixed axis * ntz + z-column tile) and issues the
+// loads of its next tile before transforming the current one, so HBM reads
+// overlap the butterflies and LDS exchanges.  The first LDS write of every
+// transform is preceded by a barrier, so consecutive tiles never race on LDS.
+template <typename T, int N, int AXIS, int MODE>
+__global__ __launch_bounds__(s_cols<T>() * (N / Sched<N>::E))
+void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw, SArgs sa, int ntz,
+                int ntiles) {
+    constexpr int E = Sched<N>::E, P = N / E, TC = s_cols<T>(), LS = lds_line<N>();
+    __shared__ cx<T> lds[TC * LS];
+    const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
+    cx<T> *L = lds + c * LS;
+    int tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    int other = sa.other0 + tile / ntz, k = (tile % ntz) * TC + c;
+    bool valid = k < kv;
+    long base, stride;
+    spass_addr<N, AXIS>(other, k, rl, base, stride);
+    cx<T> v[E], nx[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = valid ? a[base + (long)(t + e * P) * stride] : mkc<T>(0, 0);
+    for (;;) {
+        const int nt = tile + (int)gridDim.x;
+        const bool more = nt < ntiles;
+        int o2 = 0, k2 = 0;
+        bool valid2 = false;
+        long base2 = 0, stride2 = 0;
+        if (more) {
+            o2 = sa.other0 + nt / ntz;
+            k2 = (nt % ntz) * TC + c;
+            valid2 = k2 < kv;
+            spass_addr<N, AXIS>(o2, k2, rl, base2, stride2);
+#pragma unroll
+            for (int e = 0; e < E; ++e) nx[e] = valid2 ? a[base2 + (long)(t + e * P) * stride2] : mkc<T>(0, 0);
+        }
+        spass_compute<T, N, MODE>(v, L, t, tw, sa, other, k, valid);
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) a[base + (long)(t + e * P) * stride] = v[e];
+        }
+        if (!more) break;
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = nx[e];
+        tile = nt;
+        other = o2;
+        k = k2;
+        valid = valid2;
+        base = base2;
+        stride = stride2;
     }
 }
 
@@ -250,6 +333,40 @@ __global__ __launch_bounds__(256) void k_kick(cx<T> *__restrict__ psi, const T *
         cx<T> z = psi[i];
         const double rr = (double)z.x * z.x + (double)z.y * z.y;
         psi[i] = cmul(z, expmi<T>(aphi * (double)phi[i] + arho * rr));
+    }
+}
+
+// ------------------------------------------------------------------ adjoint
+// s = Im(conj(g) psi): the real field whose Green's-function image enters
+// the adjoint of a kick
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_src(const cx<T> *__restrict__ psi, const cx<T> *__restrict__ g,
+                                                 T *__restrict__ s, long n3) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
+        const cx<T> a = g[i], b = psi[i];
+        s[i] = a.x * b.y - a.y * b.x;
+    }
+}
+
+// Adjoint of the kick psi' = psi exp(-i theta), theta = aphi phi + arho |psi|^2
+// (phi = G(m_s |psi|^2 - rho_c)), applied in reverse: with psi' and the
+// gradient g' of the kicked field,
+//   psi = psi' e^{+i theta},  g = e^{+i theta} g' + 2 psi (aphi G(m_s s) + arho s),
+// s = Im(conj(g') psi'), phs = G(m_s s) (a Poisson solve of s).
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_kick(cx<T> *__restrict__ psi, cx<T> *__restrict__ g,
+                                                  const T *__restrict__ phi, const T *__restrict__ phs,
+                                                  const T *__restrict__ s, long n3, double aphi, double arho) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
+        const cx<T> p1 = psi[i], g1 = g[i];
+        const double rr = (double)p1.x * p1.x + (double)p1.y * p1.y;
+        cx<T> e = expmi<T>(aphi * (double)phi[i] + arho * rr);
+        e.y = -e.y;                                            // e^{+i theta}
+        const cx<T> p0 = cmul(p1, e);
+        const T as = (T)(aphi * (double)phs[i] + arho * (double)s[i]);
+        const cx<T> eg = cmul(e, g1);
+        psi[i] = p0;
+        g[i] = mkc<T>(eg.x + 2 * p0.x * as, eg.y + 2 * p0.y * as);
     }
 }
 

@@ -138,7 +138,10 @@ int ifdm_time_passes(ifdm_ctx *ctx, const ifdm_dt *dt, int iters, float *ms_out)
  *   "slab_planes" S: 0 (default) launches each pass over the whole grid;
  *   S > 0 runs the x-slab-local pass chains slab by slab, S x-planes per
  *   slab, so that the second pass of a chain finds its input in the
- *   memory-side cache.  Ignored while live profiling is on. */
+ *   memory-side cache.
+ *   "persistent" 0/1: the strided (y, x) passes launch one work-group per
+ *   tile (0, default) or a resident set of work-groups that walk the tiles
+ *   and prefetch the next tile's data during the current transform (1). */
 int ifdm_set_option(ifdm_ctx *ctx, const char *name, long value);
 
 /* Live per-pass timing: with on != 0 every step run by ifdm_run or
@@ -161,6 +164,31 @@ int ifdm_drift(ifdm_ctx *ctx, double dKE);
  * or +1 (FFTW_BACKWARD): the wfc_fplan / wfc_bplan executes of
  * src/CosmoBEC_MPI_stan.cpp:136,144. */
 int ifdm_fft3d(ifdm_ctx *ctx, int sign);
+
+/* ---- adjoint (no counterpart in the reference) ----------------------- */
+
+/* Forward run of nsteps steps with the given schedule (as ifdm_run) that
+ * keeps psi every K steps (step 0, K, 2K, ...) on the device for
+ * ifdm_adjoint.  FP64 contexts only. */
+int ifdm_run_checkpointed(ifdm_ctx *ctx, const ifdm_dt *sched, int nsteps, int K);
+
+/* Reverse sweep of the last ifdm_run_checkpointed (same sched, nsteps).
+ * On entry g holds dL/dRe psi_T + i dL/dIm psi_T for a real loss L of the
+ * final state (host N^3 complex128, reference layout); on return it holds
+ * dL/dRe psi_0 + i dL/dIm psi_0.  The schedule is frozen (dt does not
+ * depend on psi here).  Each kick psi' = psi e^{-i theta}, theta =
+ * aphi phi + arho |psi|^2, is reversed exactly (it preserves |psi|) with
+ * g = e^{i theta} g' + 2 psi (aphi G(m_s s) + arho s), s = Im(conj(g') psi'),
+ * G the Poisson Green's operator; each drift by its conjugate; psi is
+ * reverse-evolved and replaced by the stored checkpoint every K steps.
+ * psi holds psi_0 on return. */
+int ifdm_adjoint(ifdm_ctx *ctx, const ifdm_dt *sched, int nsteps, double *g_c128);
+/* The same with g in device memory (complex128 N^3, e.g. a torch tensor). */
+int ifdm_adjoint_device(ifdm_ctx *ctx, const ifdm_dt *sched, int nsteps, void *g_dev);
+
+/* psi from / to device memory (complex128 N^3 on the context's device). */
+int ifdm_set_psi_device(ifdm_ctx *ctx, const void *psi_dev);
+int ifdm_get_psi_device(ifdm_ctx *ctx, void *psi_dev);
 
 /* ---- initial conditions and snapshots -------------------------------- */
 

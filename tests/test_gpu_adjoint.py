@@ -1,0 +1,135 @@
+"""GPU: the hand-written adjoint of the KDK step against central finite
+differences (eps = 1e-6) along seeded random directions, through both the
+library's forward and the CPU oracle's, and through torch.autograd."""
+import numpy as np
+import pytest
+
+from tests.fields import make_psi, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _setup(n, steps, g_in=0.0, seed=3, tseed=4321):
+    import ifdm_code_amd as F
+    from oracle import oracle as O
+    rp = O.RunParams(n=n, g_in=g_in)
+    cfg = F.Config.from_cli(2.5e-24, g_in, 5.0, n)
+    psi0 = make_psi(n, seed, cfg.mfdm_solar)
+    # frozen schedule: adaptive steps of the reference loop
+    _st, used = O.simulate(rp, psi0, 1.0 / 51, steps)
+    sched = [u[:3] for u in used]
+    tgt_state, _ = O.simulate(rp, make_psi(n, tseed, cfg.mfdm_solar), 1.0 / 51, steps)
+    rho_t = np.abs(tgt_state.psi) ** 2
+    rbar = float(np.mean(np.abs(psi0) ** 2))
+    return F, O, rp, cfg, psi0, sched, rho_t, rbar
+
+
+def _loss(psi_T, rho_t, rbar):
+    return float(np.sum(((np.abs(psi_T) ** 2 - rho_t) / rbar) ** 2))
+
+
+def _seed(psi_T, rho_t, rbar):
+    return 4 * (np.abs(psi_T) ** 2 - rho_t) / rbar ** 2 * psi_T
+
+
+@pytest.mark.parametrize("n,steps,K,g_in", [(16, 12, 4, 0.0), (32, 20, 5, 4e-55)])
+def test_adjoint_matches_finite_differences(n, steps, K, g_in):
+    F, O, rp, cfg, psi0, sched, rho_t, rbar = _setup(n, steps, g_in)
+    _say(f"[adjoint] n={n} steps={steps} K={K} g={g_in}")
+    with F.Solver(cfg) as s:
+        def fwd(p):
+            s.set_psi(p)
+            s.poisson()
+            s.run(sched)
+            return s.get_psi()
+        s.set_psi(psi0)
+        s.poisson()
+        s.run_checkpointed(sched, K)
+        psiT = s.get_psi()
+        g0 = s.adjoint(sched, _seed(psiT, rho_t, rbar))
+        assert rel_l2(s.get_psi(), psi0) < 1e-11          # psi_0 restored
+        rng = np.random.default_rng(77)
+        scale = np.sqrt(rbar)
+        eps = 1e-6
+        for k in range(4):
+            d = (rng.standard_normal(psi0.shape) + 1j * rng.standard_normal(psi0.shape)) * scale / np.sqrt(2)
+            fd = (_loss(fwd(psi0 + eps * d), rho_t, rbar) - _loss(fwd(psi0 - eps * d), rho_t, rbar)) / (2 * eps)
+            ad = float(np.sum((np.conj(g0) * d).real))
+            _say(f"   dir {k}: adjoint {ad:.12e} fd {fd:.12e}")
+            assert abs(ad - fd) <= 1e-6 * abs(fd)
+
+
+def test_adjoint_against_oracle_finite_differences():
+    """Finite differences of the CPU oracle's forward (the reference's
+    unmerged step sequence) check the GPU gradient independently."""
+    n, steps = 16, 8
+    F, O, rp, cfg, psi0, sched, rho_t, rbar = _setup(n, steps, 0.0, seed=11)
+    with F.Solver(cfg) as s:
+        s.set_psi(psi0)
+        s.poisson()
+        s.run_checkpointed(sched, 4)
+        g0 = s.adjoint(sched, _seed(s.get_psi(), rho_t, rbar))
+
+    def fwd_oracle(p):
+        st = O.State(rp, p)
+        for dt in sched:
+            st.poisson()
+            st.split_op(*dt)
+        return st.psi
+    rng = np.random.default_rng(5)
+    eps = 1e-6
+    for k in range(3):
+        d = (rng.standard_normal(psi0.shape) + 1j * rng.standard_normal(psi0.shape)) * np.sqrt(rbar / 2)
+        fd = (_loss(fwd_oracle(psi0 + eps * d), rho_t, rbar) - _loss(fwd_oracle(psi0 - eps * d), rho_t, rbar)) / (2 * eps)
+        ad = float(np.sum((np.conj(g0) * d).real))
+        assert abs(ad - fd) <= 1e-6 * abs(fd)
+
+
+def test_adjoint_is_exact_for_a_pure_drift():
+    """With dPE = dQI = 0 the step is the unitary drift: the adjoint of
+    g_T is D^H g_T, i.e. the drift with -dKE applied to g_T."""
+    import ifdm_code_amd as F
+    n = 16
+    cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n)
+    psi0 = make_psi(n, 2, cfg.mfdm_solar)
+    rng = np.random.default_rng(1)
+    gT = rng.standard_normal(psi0.shape) + 1j * rng.standard_normal(psi0.shape)
+    sched = [(3e4, 0.0, 0.0), (2e4, 0.0, 0.0)]
+    with F.Solver(cfg) as s:
+        s.set_psi(psi0)
+        s.poisson()
+        s.run_checkpointed(sched, 1)
+        g0 = s.adjoint(sched, gT)
+        s.set_psi(gT)
+        s.drift(-2e4)
+        s.drift(-3e4)
+        expect = s.get_psi()
+    assert rel_l2(g0, expect) < 1e-13
+
+
+def test_torch_autograd_function():
+    import torch
+    import ifdm_code_amd as F
+    from ifdm_code_amd.autograd import evolve
+    n, steps = 16, 6
+    F_, O, rp, cfg, psi0, sched, rho_t, rbar = _setup(n, steps, 0.0, seed=21)
+    with F.Solver(cfg) as s:
+        s.set_psi(psi0)
+        s.poisson()
+        p0 = torch.tensor(psi0, dtype=torch.complex128, device="cuda:0", requires_grad=True)
+        pT = evolve(p0, s, sched, 3)
+        rt = torch.tensor(rho_t, device="cuda:0")
+        loss = (((pT.abs() ** 2) - rt) / rbar).pow(2).sum()
+        loss.backward()
+        g_torch = p0.grad.detach().cpu().numpy()
+        s.set_psi(psi0)
+        s.poisson()
+        s.run_checkpointed(sched, 3)
+        psiT = s.get_psi()
+        g_lib = s.adjoint(sched, _seed(psiT, rho_t, rbar))
+    assert rel_l2(pT.detach().cpu().numpy(), psiT) < 1e-14
+    assert rel_l2(g_torch, g_lib) < 1e-12
