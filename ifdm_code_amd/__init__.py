@@ -76,6 +76,7 @@ class _ICInfo(ctypes.Structure):
 
 
 EXPORTS = [
+    "ifdm_power_spectrum",
     "ifdm_run_checkpointed", "ifdm_adjoint", "ifdm_adjoint_device", "ifdm_set_psi_device", "ifdm_get_psi_device",
     "ifdm_read_gadget", "ifdm_ic_begin", "ifdm_ic_deposit", "ifdm_ic_finish", "ifdm_set_rhoc",
     "ifdm_write_snapshot", "ifdm_read_snapshot",
@@ -113,6 +114,7 @@ def lib():
     L.ifdm_simulate.argtypes = [vp, ctypes.c_double, ip, dp, ctypes.POINTER(_Dt)]
     L.ifdm_time_passes.argtypes = [vp, ctypes.POINTER(_Dt), ip, ctypes.POINTER(ctypes.c_float)]
     L.ifdm_fft3d.argtypes = [vp, ip]
+    L.ifdm_power_spectrum.argtypes = [vp, ip, dp, dp, dp]
     L.ifdm_run_checkpointed.argtypes = [vp, ctypes.POINTER(_Dt), ip, ip]
     L.ifdm_adjoint.argtypes = [vp, ctypes.POINTER(_Dt), ip, dp]
     L.ifdm_adjoint_device.argtypes = [vp, ctypes.POINTER(_Dt), ip, vp]
@@ -258,6 +260,14 @@ class Solver:
         if a.size != self.cfg.n ** 3:
             raise ValueError("phi size mismatch")
         self._chk(lib().ifdm_set_phi(self._h, _dp(a)), "ifdm_set_phi")
+
+    # ---- diagnostics
+    def power_spectrum(self, nbins: int | None = None):
+        """(k [1/Mpc], P(k) [Mpc^3], modes) on shells of width 2 pi / L."""
+        nb = nbins or self.cfg.n // 2
+        k, pk, nm = np.zeros(nb), np.zeros(nb), np.zeros(nb)
+        self._chk(lib().ifdm_power_spectrum(self._h, nb, _dp(k), _dp(pk), _dp(nm)), "ifdm_power_spectrum")
+        return k, pk, nm
 
     # ---- adjoint
     @staticmethod

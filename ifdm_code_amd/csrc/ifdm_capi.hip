@@ -158,6 +158,23 @@ struct Ops {
         return 0;
     }
 
+    // forward 3-D real transform of m_s|psi|^2 - rho_c into shat, binned
+    static int power_spectrum(ifdm_ctx *c, int nbins, double *dev_out) {
+        const SArgs sa = sargs(c, 0.0);
+        int rc;
+        if ((rc = zpass<Z_SRC>(c, zargs(c, 0, 0), c->part_rho))) return rc;
+        if ((rc = spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sa))) return rc;
+        if ((rc = spass<0, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sa))) return rc;
+        HIPCHK(c, hipMemsetAsync(dev_out, 0, sizeof(double) * 3 * nbins, c->stream));
+        const long nh = (long)N * N * (N / 2 + 1);
+        const unsigned nb = (unsigned)std::min<long>((nh + 255) / 256, 2048);
+        hipLaunchKernelGGL((k_pk_bin<T, N>), dim3(nb), dim3(256), sizeof(double) * 3 * nbins, c->stream,
+                           (const cx<T> *)c->shat, c->hs, nbins, dev_out);
+        LAUNCHCHK(c);
+        c->cur_pass = 9;
+        return pass(c, 9, 0, 0, 0);   // reductions (the norm gives the mean density)
+    }
+
     static int poisson(ifdm_ctx *c) {
         int rc;
         if ((rc = zpass<Z_SRC>(c, zargs(c, 0, 0), c->part_rho))) return rc;
@@ -266,6 +283,7 @@ struct Ops {
 };
 
 struct Dispatch {
+    int (*power_spectrum)(ifdm_ctx *, int, double *);
     int (*adj_kick)(ifdm_ctx *, double, double);
     int (*theta_solve)(ifdm_ctx *, double);
     int (*pass)(ifdm_ctx *, int, double, double, double);
@@ -278,7 +296,7 @@ struct Dispatch {
 
 template <typename T, int N>
 Dispatch make_dispatch() {
-    return Dispatch{&Ops<T, N>::adj_kick, &Ops<T, N>::theta_solve, &Ops<T, N>::pass, &Ops<T, N>::poisson, &Ops<T, N>::kick_drift_poisson, &Ops<T, N>::drift, &Ops<T, N>::fft3d,
+    return Dispatch{&Ops<T, N>::power_spectrum, &Ops<T, N>::adj_kick, &Ops<T, N>::theta_solve, &Ops<T, N>::pass, &Ops<T, N>::poisson, &Ops<T, N>::kick_drift_poisson, &Ops<T, N>::drift, &Ops<T, N>::fft3d,
                     &Ops<T, N>::kick};
 }
 
@@ -700,6 +718,38 @@ int ifdm_internal_theta_solve(ifdm_ctx *c, void *buf64, double coef) {
     hipFree(sh.tw);
     sh.stream = nullptr;
     return rc;
+}
+
+// ------------------------------------------------------------ diagnostics
+int ifdm_power_spectrum(ifdm_ctx *c, int nbins, double *k_mean, double *pk, double *nmodes) {
+    if (!c || nbins <= 0 || nbins > 4096 || !pk) return IFDM_E_ARG;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    Dispatch d;
+    get_dispatch(c->n, c->cfg.precision, &d);
+    double *dev = nullptr;
+    HIPCHK(c, hipMalloc((void **)&dev, sizeof(double) * 3 * nbins));
+    int rc = d.power_spectrum(c, nbins, dev);
+    std::vector<double> h(3 * (size_t)nbins);
+    ifdm_stats st;
+    if (!rc) rc = read_stats(c, &st);
+    if (!rc && hipMemcpy(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+        c->err = "power spectrum copy failed";
+        rc = IFDM_E_HIP;
+    }
+    hipFree(dev);
+    c->ev_pass.clear();
+    if (rc) return rc;
+    // P(k) = L^3 |delta_k|^2 / N^6 with delta = (m_s|psi|^2 - rho_mean) / rho_mean
+    const double L = 2.0 * c->cfg.xmax, n6 = std::pow((double)c->n, 6);
+    const double rho_mean = c->ms * st.norm / (double)c->n3;
+    const double kf = 2.0 * M_PI / L;
+    for (int b = 0; b < nbins; ++b) {
+        const double w = h[3 * b + 2];
+        pk[b] = w > 0 ? std::pow(L, 3) * h[3 * b] / (rho_mean * rho_mean * n6) / w : 0.0;
+        if (k_mean) k_mean[b] = w > 0 ? kf * h[3 * b + 1] / w : kf * (b + 1);
+        if (nmodes) nmodes[b] = w;
+    }
+    return IFDM_OK;
 }
 
 // ---------------------------------------------------------------- adjoint

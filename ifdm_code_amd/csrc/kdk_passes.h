@@ -370,6 +370,38 @@ __global__ __launch_bounds__(256) void k_adj_kick(cx<T> *__restrict__ psi, cx<T>
     }
 }
 
+// ------------------------------------------------------------ diagnostics
+// Matter power spectrum from the forward 3-D half spectrum of the Poisson
+// source: per |n|-shell (n = integer wave vector, bin b = round(|n|),
+// b = 1..nbins) the sums of w |s_k|^2, w |n| and w, with w = 1 on the
+// kz = 0 and kz = N/2 planes and 2 elsewhere (Hermitian partners).  Each
+// block bins into LDS, then adds its bins to the global sums.
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_pk_bin(const cx<T> *__restrict__ shat, int hs, int nbins,
+                                                double *__restrict__ out) {
+    extern __shared__ double sb[];   // 3 * nbins
+    for (int i = threadIdx.x; i < 3 * nbins; i += blockDim.x) sb[i] = 0.0;
+    __syncthreads();
+    const long nh = (long)N * N * (N / 2 + 1);
+    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < nh; o += (long)gridDim.x * blockDim.x) {
+        const int kz = (int)(o % (N / 2 + 1));
+        const long r = o / (N / 2 + 1);
+        const int j = (int)(r % N), i = (int)(r / N);
+        const int nx = i < N / 2 ? i : i - N, ny = j < N / 2 ? j : j - N;
+        const double nn = sqrt((double)nx * nx + (double)ny * ny + (double)kz * kz);
+        const int b = (int)(nn + 0.5);
+        if (b < 1 || b > nbins) continue;
+        const cx<T> v = shat[r * hs + kz];
+        const double w = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
+        atomicAdd(&sb[3 * (b - 1)], w * ((double)v.x * v.x + (double)v.y * v.y));
+        atomicAdd(&sb[3 * (b - 1) + 1], w * nn);
+        atomicAdd(&sb[3 * (b - 1) + 2], w);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * nbins; i += blockDim.x)
+        if (sb[i] != 0.0) atomicAdd(&out[i], sb[i]);
+}
+
 // plain 16-byte-vector copy: the achievable streaming bandwidth reference
 __global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long n) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)

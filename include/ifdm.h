@@ -165,6 +165,16 @@ int ifdm_drift(ifdm_ctx *ctx, double dKE);
  * src/CosmoBEC_MPI_stan.cpp:136,144. */
 int ifdm_fft3d(ifdm_ctx *ctx, int sign);
 
+/* ---- diagnostics (no counterpart in the reference) ------------------- */
+
+/* Matter power spectrum of delta = m_s|psi|^2 / rho_mean - 1 on shells of
+ * width 2 pi / L: bin b (0-based) holds the modes with round(|n|) = b + 1,
+ * n the integer wave vector.  pk[b] = L^3 <|delta_k|^2> / N^6 (Mpc^3),
+ * k_mean[b] the mode-weighted mean |k| (1/Mpc), nmodes[b] the number of
+ * modes (both halves of the Hermitian spectrum counted).  k_mean and
+ * nmodes may be NULL.  Uses the Poisson scratch; psi and phi are kept. */
+int ifdm_power_spectrum(ifdm_ctx *ctx, int nbins, double *k_mean, double *pk, double *nmodes);
+
 /* ---- adjoint (no counterpart in the reference) ----------------------- */
 
 /* Forward run of nsteps steps with the given schedule (as ifdm_run) that
