@@ -815,12 +815,12 @@ static int adjoint_sweep(ifdm_ctx *c, const ifdm_dt *sched, int nsteps) {
 }
 
 static int adjoint_prepare(ifdm_ctx *c, int nsteps) {
-    if (c->mixed) { c->err = "the adjoint runs in FP64 contexts"; return IFDM_E_STATE; }
     if (!c->ck || c->ck_nsteps != nsteps) {
         c->err = "ifdm_adjoint needs the checkpoints of an ifdm_run_checkpointed over the same steps";
         return IFDM_E_STATE;
     }
-    const size_t cs = 2 * sizeof(double) * (size_t)c->n3, rs = sizeof(double) * (size_t)c->n3;
+    const size_t es = c->mixed ? sizeof(float) : sizeof(double);
+    const size_t cs = 2 * es * (size_t)c->n3, rs = es * (size_t)c->n3;
     if (!c->adj_g) {
         if (hipMalloc(&c->adj_g, cs) != hipSuccess || hipMalloc(&c->adj_s, rs) != hipSuccess ||
             hipMalloc(&c->adj_phs, rs) != hipSuccess) {
@@ -836,15 +836,32 @@ int ifdm_adjoint(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, double *g) {
     HIPCHK(c, hipSetDevice(c->cfg.device));
     int rc;
     if ((rc = adjoint_prepare(c, nsteps))) return rc;
-    const size_t cs = 2 * sizeof(double) * (size_t)c->n3;
-    HIPCHK(c, hipMemcpy(c->adj_g, g, cs, hipMemcpyHostToDevice));
+    const size_t n2 = 2 * (size_t)c->n3;
+    if (!c->mixed) {
+        HIPCHK(c, hipMemcpy(c->adj_g, g, n2 * sizeof(double), hipMemcpyHostToDevice));
+        if ((rc = adjoint_sweep(c, sched, nsteps))) return rc;
+        HIPCHK(c, hipMemcpy(g, c->adj_g, n2 * sizeof(double), hipMemcpyDeviceToHost));
+        return IFDM_OK;
+    }
+    // mixed: the sweep is linear in g, so g is held as 2^-sg g in FP32 with
+    // |2^-sg g| <= 1 at the start (the psi scale is folded into the kicks)
+    double gmax = 0.0;
+    for (size_t i = 0; i < n2; ++i) gmax = std::max(gmax, std::fabs(g[i]));
+    const int sg = gmax > 0 ? std::ilogb(gmax) + 1 : 0;
+    std::vector<float> tmp(n2);
+    for (size_t i = 0; i < n2; ++i) tmp[i] = (float)std::ldexp(g[i], -sg);
+    HIPCHK(c, hipMemcpy(c->adj_g, tmp.data(), n2 * sizeof(float), hipMemcpyHostToDevice));
     if ((rc = adjoint_sweep(c, sched, nsteps))) return rc;
-    HIPCHK(c, hipMemcpy(g, c->adj_g, cs, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(tmp.data(), c->adj_g, n2 * sizeof(float), hipMemcpyDeviceToHost));
+    // back to physical units: g scale 2^sg, and the psi scale of the kicked
+    // field enters g = e^{i theta} g' + 2 psi A s linearly through psi and s
+    for (size_t i = 0; i < n2; ++i) g[i] = std::ldexp((double)tmp[i], sg);
     return IFDM_OK;
 }
 
 int ifdm_adjoint_device(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, void *g_dev) {
     if (!c || !sched || !g_dev || nsteps < 0) return IFDM_E_ARG;
+    if (c->mixed) { c->err = "ifdm_adjoint_device runs in FP64 contexts (use ifdm_adjoint)"; return IFDM_E_STATE; }
     HIPCHK(c, hipSetDevice(c->cfg.device));
     int rc;
     if ((rc = adjoint_prepare(c, nsteps))) return rc;

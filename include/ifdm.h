@@ -179,7 +179,8 @@ int ifdm_power_spectrum(ifdm_ctx *ctx, int nbins, double *k_mean, double *pk, do
 
 /* Forward run of nsteps steps with the given schedule (as ifdm_run) that
  * keeps psi every K steps (step 0, K, 2K, ...) on the device for
- * ifdm_adjoint.  FP64 contexts only. */
+ * ifdm_adjoint.  In mixed contexts the checkpoints are FP32 and
+ * ifdm_adjoint holds g as 2^-s g in FP32 (the sweep is linear in g). */
 int ifdm_run_checkpointed(ifdm_ctx *ctx, const ifdm_dt *sched, int nsteps, int K);
 
 /* Reverse sweep of the last ifdm_run_checkpointed (same sched, nsteps).
@@ -193,7 +194,8 @@ int ifdm_run_checkpointed(ifdm_ctx *ctx, const ifdm_dt *sched, int nsteps, int K
  * reverse-evolved and replaced by the stored checkpoint every K steps.
  * psi holds psi_0 on return. */
 int ifdm_adjoint(ifdm_ctx *ctx, const ifdm_dt *sched, int nsteps, double *g_c128);
-/* The same with g in device memory (complex128 N^3, e.g. a torch tensor). */
+/* The same with g in device memory (complex128 N^3, e.g. a torch tensor);
+ * FP64 contexts. */
 int ifdm_adjoint_device(ifdm_ctx *ctx, const ifdm_dt *sched, int nsteps, void *g_dev);
 
 /* psi from / to device memory (complex128 N^3 on the context's device). */

@@ -133,3 +133,23 @@ def test_torch_autograd_function():
         g_lib = s.adjoint(sched, _seed(psiT, rho_t, rbar))
     assert rel_l2(pT.detach().cpu().numpy(), psiT) < 1e-14
     assert rel_l2(g_torch, g_lib) < 1e-12
+
+
+def test_mixed_adjoint_against_fp64():
+    """The mixed-precision adjoint (FP32 fields, power-of-two scaled psi and
+    g, FP64 phases) against the FP64 adjoint on the same schedule."""
+    import ifdm_code_amd as F
+    n, steps = 32, 10
+    F_, O, rp, cfg, psi0, sched, rho_t, rbar = _setup(n, steps, 4e-55, seed=8)
+    grads = []
+    for prec in (0, 1):
+        c = F.Config.from_cli(2.5e-24, 4e-55, 5.0, n, precision=prec)
+        with F.Solver(c) as s:
+            s.set_psi(psi0)
+            s.poisson()
+            s.run_checkpointed(sched, 5)
+            psiT = s.get_psi()
+            grads.append(s.adjoint(sched, _seed(psiT, rho_t, rbar)))
+    err = rel_l2(grads[1], grads[0])
+    _say(f"   mixed vs FP64 adjoint gradient rel L2 = {err:.3e}")
+    assert err < 1e-4
