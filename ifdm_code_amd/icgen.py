@@ -35,39 +35,48 @@ def growth_rate(a: float) -> float:
 
 
 def zeldovich(n: int, box_mpc_h: float, z: float = 50.0, seed: int = 1234, slope: float = -2.5,
-              rms_cells: float = 0.2):
+              rms_cells: float = 0.2, dtype=None):
     """Returns (pos, vel, pmass, a): pos and vel float32 arrays (n^3, 3) in
-    kpc/h and Gadget km/s, the particle mass in 1e10 Msun/h, and a."""
+    kpc/h and Gadget km/s, the particle mass in 1e10 Msun/h, and a.  The
+    field is built in float64 up to 256^3 and in float32 above (memory:
+    about 10 x 4 n^3 bytes at 1024^3)."""
+    if dtype is None:
+        dtype = np.float64 if n <= 256 else np.float32
     a = 1.0 / (1.0 + z)
     box = box_mpc_h * 1e3                       # kpc/h
     lg = box / n
     rng = np.random.default_rng(seed)
-    k1 = np.fft.fftfreq(n) * n                  # integer wave numbers
-    kz = np.arange(n // 2 + 1, dtype=np.float64)
-    kx, ky = k1[:, None, None], k1[None, :, None]
-    kk = kx ** 2 + ky ** 2 + kz[None, None, :] ** 2
-    amp = np.zeros_like(kk)
+    k1 = (np.fft.fftfreq(n) * n).astype(dtype)  # integer wave numbers
+    kz = np.arange(n // 2 + 1, dtype=dtype)
+    kk = k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + kz[None, None, :] ** 2
     nz = (kk > 0) & (kk <= (n / 2) ** 2)
-    amp[nz] = kk[nz] ** (slope / 4)
-    dk = np.fft.rfftn(rng.standard_normal((n, n, n))) * amp
+    amp = np.zeros_like(kk)
+    amp[nz] = kk[nz] ** dtype(slope / 4)
+    del nz
+    dk = np.fft.rfftn(rng.standard_normal((n, n, n), dtype=dtype)) * amp
     del amp
     kk[0, 0, 0] = 1.0
-    disp = []
-    for kc in (np.broadcast_to(kx, kk.shape), np.broadcast_to(ky, kk.shape), np.broadcast_to(kz[None, None, :], kk.shape)):
-        # s_k = i k delta_k / k^2 (grid units: k in units of 2 pi / L)
-        disp.append(np.fft.irfftn(1j * kc * dk / kk, s=(n, n, n), axes=(0, 1, 2)))
-    del dk, kk
-    s = np.stack(disp, axis=-1).reshape(-1, 3)
-    del disp
-    rms = math.sqrt(float(np.mean(np.sum(s * s, axis=1))))
-    s *= rms_cells * lg / rms                   # kpc/h
-    i = (np.arange(n) + 0.5) * lg
-    q = np.stack(np.meshgrid(i, i, i, indexing="ij"), axis=-1).reshape(-1, 3)
-    pos = np.mod(q + s, box).astype(np.float32)
-    pos[pos >= box] = 0.0                       # float rounding at the upper edge
+    dk /= kk                                    # delta_k / k^2
+    del kk
+    # s_k = i k delta_k / k^2, one axis at a time
+    s = np.empty((n ** 3, 3), np.float32)
+    for ax in range(3):
+        kc = (k1[:, None, None], k1[None, :, None], kz[None, None, :])[ax]
+        s[:, ax] = np.fft.irfftn(1j * kc * dk, s=(n, n, n), axes=(0, 1, 2)).ravel()
+    del dk
+    rms = math.sqrt(float(np.mean(np.sum(s.astype(np.float64) ** 2, axis=1))))
+    s *= np.float32(rms_cells * lg / rms)       # kpc/h
     e = math.sqrt(OMEGA_M / a ** 3 + 1 - OMEGA_M)
-    v = a * 0.1 * e * growth_rate(a) * s        # km/s (100 h E(a) km/s/Mpc x s[kpc/h] / (1e3 h))
-    vel = (v / math.sqrt(a)).astype(np.float32)
+    # v = a H f s: 100 h E(a) km/s/Mpc x s[kpc/h] / (1e3 h); Gadget u = v / sqrt(a)
+    vel = (s * np.float32(a * 0.1 * e * growth_rate(a) / math.sqrt(a))).astype(np.float32)
+    idx = (np.arange(n, dtype=np.float64) + 0.5) * lg
+    pos = s.reshape(n, n, n, 3)
+    pos[..., 0] += idx[:, None, None].astype(np.float32)
+    pos[..., 1] += idx[None, :, None].astype(np.float32)
+    pos[..., 2] += idx[None, None, :].astype(np.float32)
+    pos = pos.reshape(-1, 3)
+    np.mod(pos, np.float32(box), out=pos)
+    pos[pos >= box] = 0.0                       # float rounding at the upper edge
     pmass = RHOC * box ** 3 / (n ** 3 * 1e10 * 1e9 * H0_LITTLE ** 2)
     return pos, vel, pmass, a
 
