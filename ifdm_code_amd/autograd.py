@@ -6,6 +6,10 @@ reverse sweep (ifdm_adjoint_device) with checkpoints every K steps.  psi0 is
 a complex128 CUDA tensor on the solver's device, shape (N, N, N).  Gradients
 follow torch's convention for real losses of complex tensors,
 dL/dRe + i dL/dIm, which is the convention of ifdm_adjoint.
+
+Import torch (and initialise its GPU context) before the library is first
+used in a process: torch ships its own HIP runtime with the same soname as
+the system one libifdm.so links, and whichever is loaded first serves both.
 """
 from __future__ import annotations
 

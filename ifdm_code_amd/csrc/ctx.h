@@ -54,8 +54,7 @@ struct ifdm_ctx {
     // launch schedule: 0 = one launch per pass; S > 0 = the x-slab-local
     // chains (A,B), (D,E,F), (H,I) run slab by slab, S x-planes per slab
     int slab = 0;
-    // strided passes: 0 = one tile per work-group; 1 = persistent
-    // work-groups with next-tile prefetch
+    // strided passes with persistent prefetching work-groups: bit 0 y, bit 1 x
     int persistent = 0;
     int num_cu = 0;
     // optional live per-pass timing (ifdm_set_profiling)

@@ -102,11 +102,51 @@ template <int R, int DIR, typename T> __device__ __forceinline__ void dft(cx<T> 
 }
 
 // ---------------------------------------------------------------- line FFT
+// Base twiddles of one thread: for stage S >= 1 and its butterfly q
+// (j = t + q*P) the factor W_N^((j mod Ns) * N/(Ns*R)); the powers
+// W^r, r < R, are formed by complex products of depth <= 3 (r = 2: w^2;
+// 3: w^2 w; 4: (w^2)^2; 5: w^4 w; 6: w^4 w^2; 7: w^4 w^3).  The base
+// values are loaded once per thread at the start of a pass, so their
+// latency hides behind the data loads instead of stalling each stage.
+template <typename T, int N> struct TwBase {
+    static constexpr int E = Sched<N>::E;
+    cx<T> w[4][E];  // [stage][butterfly]; stage 0 unused
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void load_twiddles(TwBase<T, N> &b, int t, const cx<T> *__restrict__ tw) {
+    constexpr int E = Sched<N>::E, P = N / E;
+#pragma unroll
+    for (int S = 1; S < Sched<N>::ns; ++S) {
+        const int R = Sched<N>::r[S], Q = E / R, NS = ns_before<N>(S);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) b.w[S][q] = tw[((t + q * P) & (NS - 1)) * (N / (NS * R))];
+    }
+}
+
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void apply_twiddles(cx<T> *a, cx<T> w1) {
+    if constexpr (DIR > 0) w1.y = -w1.y;
+    a[1] = cmul(a[1], w1);
+    if constexpr (R >= 4) {
+        const cx<T> w2 = cmul(w1, w1);
+        a[2] = cmul(a[2], w2);
+        a[3] = cmul(a[3], cmul(w2, w1));
+        if constexpr (R == 8) {
+            const cx<T> w4 = cmul(w2, w2), w3 = cmul(w2, w1);
+            a[4] = cmul(a[4], w4);
+            a[5] = cmul(a[5], cmul(w4, w1));
+            a[6] = cmul(a[6], cmul(w4, w2));
+            a[7] = cmul(a[7], cmul(w4, w3));
+        }
+    }
+}
+
 // One Stockham stage S of an N-point transform, direction DIR (-1 forward,
 // +1 inverse), for the thread at line position t.  L is the line's LDS
-// region; tw the table W_N^k = exp(-2 pi i k / N), k < N.
+// region; b the thread's base twiddles.
 template <typename T, int N, int DIR, int S>
-__device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const cx<T> *__restrict__ tw) {
+__device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &b) {
     constexpr int E = Sched<N>::E, P = N / E, R = Sched<N>::r[S], Q = E / R;
     constexpr int NS = ns_before<N>(S);
     constexpr bool FIRST = (S == 0), LAST = (S == Sched<N>::ns - 1);
@@ -119,15 +159,7 @@ __device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const cx<T>
             if constexpr (FIRST) a[q][r] = v[q + r * Q];
             else a[q][r] = L[lpos(j + r * (N / R))];
         }
-        if constexpr (!FIRST) {
-            const int k = j & (NS - 1);
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                cx<T> w = tw[r * k * (N / (NS * R))];
-                if constexpr (DIR > 0) w.y = -w.y;
-                a[q][r] = cmul(a[q][r], w);
-            }
-        }
+        if constexpr (!FIRST) apply_twiddles<R, DIR>(a[q], b.w[S][q]);
         dft<R, DIR>(a[q]);
     }
     if constexpr (LAST) {
@@ -149,11 +181,19 @@ __device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const cx<T>
 }
 
 template <typename T, int N, int DIR>
+__device__ __forceinline__ void fft_line(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &b) {
+    fft_stage<T, N, DIR, 0>(v, L, t, b);
+    if constexpr (Sched<N>::ns > 1) fft_stage<T, N, DIR, 1>(v, L, t, b);
+    if constexpr (Sched<N>::ns > 2) fft_stage<T, N, DIR, 2>(v, L, t, b);
+    if constexpr (Sched<N>::ns > 3) fft_stage<T, N, DIR, 3>(v, L, t, b);
+}
+
+// convenience: load the base twiddles and transform
+template <typename T, int N, int DIR>
 __device__ __forceinline__ void fft_line(cx<T> *v, cx<T> *L, int t, const cx<T> *__restrict__ tw) {
-    fft_stage<T, N, DIR, 0>(v, L, t, tw);
-    if constexpr (Sched<N>::ns > 1) fft_stage<T, N, DIR, 1>(v, L, t, tw);
-    if constexpr (Sched<N>::ns > 2) fft_stage<T, N, DIR, 2>(v, L, t, tw);
-    if constexpr (Sched<N>::ns > 3) fft_stage<T, N, DIR, 3>(v, L, t, tw);
+    TwBase<T, N> b;
+    load_twiddles<T, N>(b, t, tw);
+    fft_line<T, N, DIR>(v, L, t, b);
 }
 
 }  // namespace ifdm

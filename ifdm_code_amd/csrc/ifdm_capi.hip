@@ -75,7 +75,7 @@ struct Ops {
         const int ntz = (kv + TC - 1) / TC;
         int rc;
         if ((rc = prof_begin(c))) return rc;
-        if (c->persistent) {
+        if (c->persistent & (AXIS == 1 ? 1 : 2)) {
             static int occ = -1;   // resident work-groups per CU for this instantiation
             if (occ < 0) {
                 int o = 0;
@@ -951,7 +951,7 @@ int ifdm_set_option(ifdm_ctx *c, const char *name, long value) {
         return IFDM_OK;
     }
     if (std::strcmp(name, "persistent") == 0) {
-        if (value < 0 || value > 1) { c->err = "persistent must be 0 or 1"; return IFDM_E_ARG; }
+        if (value < 0 || value > 3) { c->err = "persistent must be a mask in 0..3"; return IFDM_E_ARG; }
         c->persistent = (int)value;
         return IFDM_OK;
     }

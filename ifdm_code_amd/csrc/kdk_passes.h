@@ -109,6 +109,8 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
     const bool valid = line < (long)N * N;
     cx<T> *L = lds + l * LS;
     cx<T> v[E];
+    TwBase<T, N> tb;
+    load_twiddles<T, N>(tb, t, tw);
 
     if constexpr (MODE == Z_C2R) {
         const cx<T> *row = shat + line * HS;
@@ -122,7 +124,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             }
             v[e] = s;
         }
-        fft_line<T, N, +1>(v, L, t, tw);
+        fft_line<T, N, +1>(v, L, t, tb);
         double mx = 0.0;
         T *out = phi + line * N;
 #pragma unroll
@@ -144,7 +146,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         cx<T> s[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) s[e] = mkc<T>(valid ? (T)(za.ms * (double)rrow[t + e * P]) : (T)0, (T)0);
-        fft_line<T, N, -1>(s, L, t, tw);
+        fft_line<T, N, -1>(s, L, t, tb);
         if (valid) {
             cx<T> *srow = shat + line * HS;
 #pragma unroll
@@ -170,7 +172,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         }
         if constexpr (MODE == Z_FWD || MODE == Z_KICK_FWD) {
             if (za.dtab && blk == 0) write_drift_table<T, N>((cx<T> *)za.dtab, za.k2, za.ke_dke, tid, NT);
-            fft_line<T, N, -1>(v, L, t, tw);
+            fft_line<T, N, -1>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -178,7 +180,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             return;
         }
         if constexpr (MODE == Z_INV || MODE == Z_INV_SRC) {
-            fft_line<T, N, +1>(v, L, t, tw);
+            fft_line<T, N, +1>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -193,7 +195,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
                 if (valid) { mx = fmax(mx, rr); sm += rr; }
                 s[e] = mkc<T>(valid ? (T)(za.ms * rr - za.rhoc) : (T)0, (T)0);
             }
-            fft_line<T, N, -1>(s, L, t, tw);
+            fft_line<T, N, -1>(s, L, t, tb);
             if (valid) {
                 cx<T> *srow = shat + line * HS;
 #pragma unroll
@@ -217,7 +219,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
 
 // the transform (and the spectral multiply of passes C and G) of one tile
 template <typename T, int N, int MODE>
-__device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const cx<T> *__restrict__ tw,
+__device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &tw,
                                               const SArgs &sa, int other, int k, bool valid) {
     constexpr int E = Sched<N>::E, P = N / E;
     if constexpr (MODE == S_FWD) fft_line<T, N, -1>(v, L, t, tw);
@@ -249,6 +251,29 @@ __device__ __forceinline__ void spass_addr(int other, int k, int rl, long &base,
     else { base = (long)other * rl + k; stride = (long)N * rl; }
 }
 
+// Element access within one tile: a wave-uniform tile base (scalar
+// registers) plus a per-lane 32-bit byte offset, which maps to the
+// scalar-base + vector-offset form of global loads and stores.  Up to
+// N = 512 every offset inside an array fits 32 bits; N = 1024 uses 64-bit
+// addresses.
+template <typename T, int N> struct TileIO {
+    static constexpr bool k32 = N <= 512;
+    const char *base;   // wave-uniform
+    __device__ __forceinline__ TileIO(const cx<T> *a, long uniform_elem) {
+        const long u = __builtin_amdgcn_readfirstlane((int)(uniform_elem >> 32)) * (1L << 32) +
+                       (unsigned)__builtin_amdgcn_readfirstlane((int)(uniform_elem & 0xffffffffL));
+        base = reinterpret_cast<const char *>(a + u);
+    }
+    __device__ __forceinline__ cx<T> load(long elem) const {
+        if constexpr (k32) return *reinterpret_cast<const cx<T> *>(base + (unsigned)((unsigned)elem * sizeof(cx<T>)));
+        else return reinterpret_cast<const cx<T> *>(base)[elem];
+    }
+    __device__ __forceinline__ void store(long elem, cx<T> v) const {
+        if constexpr (k32) *reinterpret_cast<cx<T> *>(const_cast<char *>(base) + (unsigned)((unsigned)elem * sizeof(cx<T>))) = v;
+        else reinterpret_cast<cx<T> *>(const_cast<char *>(base))[elem] = v;
+    }
+};
+
 // one tile per work-group: grid (z-column tiles, lines of the fixed axis)
 template <typename T, int N, int AXIS, int MODE>
 __global__ __launch_bounds__(s_cols<T>() * (N / Sched<N>::E))
@@ -256,19 +281,22 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
     constexpr int E = Sched<N>::E, P = N / E, TC = s_cols<T>(), LS = lds_line<N>();
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
-    const int k = blockIdx.x * TC + c;
+    const int k0 = blockIdx.x * TC, k = k0 + c;
     const int other = sa.other0 + (int)blockIdx.y;
     const bool valid = k < kv;
-    long base, stride;
-    spass_addr<N, AXIS>(other, k, rl, base, stride);
+    long base0, stride;
+    spass_addr<N, AXIS>(other, k0, rl, base0, stride);
+    const TileIO<T, N> io(a, base0);
     cx<T> *L = lds + c * LS;
+    TwBase<T, N> tb;
+    load_twiddles<T, N>(tb, t, tw);
     cx<T> v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = valid ? a[base + (long)(t + e * P) * stride] : mkc<T>(0, 0);
-    spass_compute<T, N, MODE>(v, L, t, tw, sa, other, k, valid);
+    for (int e = 0; e < E; ++e) v[e] = valid ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
+    spass_compute<T, N, MODE>(v, L, t, tb, sa, other, k, valid);
     if (valid) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) a[base + (long)(t + e * P) * stride] = v[e];
+        for (int e = 0; e < E; ++e) io.store(c + (long)(t + e * P) * stride, v[e]);
     }
 }
 
@@ -287,41 +315,47 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__
     cx<T> *L = lds + c * LS;
     int tile = blockIdx.x;
     if (tile >= ntiles) return;
-    int other = sa.other0 + tile / ntz, k = (tile % ntz) * TC + c;
-    bool valid = k < kv;
-    long base, stride;
-    spass_addr<N, AXIS>(other, k, rl, base, stride);
+    const long stride = AXIS == 1 ? (long)rl : (long)N * rl;
+    TwBase<T, N> tb;
+    load_twiddles<T, N>(tb, t, tw);
     cx<T> v[E], nx[E];
+    {
+        const int other = sa.other0 + tile / ntz, k0 = (tile % ntz) * TC;
+        long b0, st;
+        spass_addr<N, AXIS>(other, k0, rl, b0, st);
+        const TileIO<T, N> io(a, b0);
+        const bool valid = k0 + c < kv;
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = valid ? a[base + (long)(t + e * P) * stride] : mkc<T>(0, 0);
+        for (int e = 0; e < E; ++e) v[e] = valid ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
+    }
     for (;;) {
         const int nt = tile + (int)gridDim.x;
         const bool more = nt < ntiles;
-        int o2 = 0, k2 = 0;
-        bool valid2 = false;
-        long base2 = 0, stride2 = 0;
         if (more) {
-            o2 = sa.other0 + nt / ntz;
-            k2 = (nt % ntz) * TC + c;
-            valid2 = k2 < kv;
-            spass_addr<N, AXIS>(o2, k2, rl, base2, stride2);
+            const int o2 = sa.other0 + nt / ntz, k2 = (nt % ntz) * TC;
+            long b2, st;
+            spass_addr<N, AXIS>(o2, k2, rl, b2, st);
+            const TileIO<T, N> io(a, b2);
+            const bool valid2 = k2 + c < kv;
 #pragma unroll
-            for (int e = 0; e < E; ++e) nx[e] = valid2 ? a[base2 + (long)(t + e * P) * stride2] : mkc<T>(0, 0);
+            for (int e = 0; e < E; ++e) nx[e] = valid2 ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
         }
-        spass_compute<T, N, MODE>(v, L, t, tw, sa, other, k, valid);
-        if (valid) {
+        const int other = sa.other0 + tile / ntz, k0 = (tile % ntz) * TC, k = k0 + c;
+        const bool valid = k < kv;
+        spass_compute<T, N, MODE>(v, L, t, tb, sa, other, k, valid);
+        {
+            long b0, st;
+            spass_addr<N, AXIS>(other, k0, rl, b0, st);
+            const TileIO<T, N> io(a, b0);
+            if (valid) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) a[base + (long)(t + e * P) * stride] = v[e];
+                for (int e = 0; e < E; ++e) io.store(c + (long)(t + e * P) * stride, v[e]);
+            }
         }
         if (!more) break;
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = nx[e];
         tile = nt;
-        other = o2;
-        k = k2;
-        valid = valid2;
-        base = base2;
-        stride = stride2;
     }
 }
 

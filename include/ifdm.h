@@ -139,9 +139,10 @@ int ifdm_time_passes(ifdm_ctx *ctx, const ifdm_dt *dt, int iters, float *ms_out)
  *   S > 0 runs the x-slab-local pass chains slab by slab, S x-planes per
  *   slab, so that the second pass of a chain finds its input in the
  *   memory-side cache.
- *   "persistent" 0/1: the strided (y, x) passes launch one work-group per
- *   tile (0, default) or a resident set of work-groups that walk the tiles
- *   and prefetch the next tile's data during the current transform (1). */
+ *   "persistent" mask: bit 0 for the y passes, bit 1 for the x passes; a set
+ *   bit launches a resident set of work-groups that walk the tiles and
+ *   prefetch the next tile's data during the current transform instead of
+ *   one work-group per tile. */
 int ifdm_set_option(ifdm_ctx *ctx, const char *name, long value);
 
 /* Live per-pass timing: with on != 0 every step run by ifdm_run or

@@ -111,28 +111,52 @@ def test_adjoint_is_exact_for_a_pure_drift():
     assert rel_l2(g0, expect) < 1e-13
 
 
-def test_torch_autograd_function():
-    import torch
-    import ifdm_code_amd as F
-    from ifdm_code_amd.autograd import evolve
-    n, steps = 16, 6
-    F_, O, rp, cfg, psi0, sched, rho_t, rbar = _setup(n, steps, 0.0, seed=21)
-    with F.Solver(cfg) as s:
-        s.set_psi(psi0)
-        s.poisson()
-        p0 = torch.tensor(psi0, dtype=torch.complex128, device="cuda:0", requires_grad=True)
-        pT = evolve(p0, s, sched, 3)
-        rt = torch.tensor(rho_t, device="cuda:0")
-        loss = (((pT.abs() ** 2) - rt) / rbar).pow(2).sum()
-        loss.backward()
-        g_torch = p0.grad.detach().cpu().numpy()
-        s.set_psi(psi0)
-        s.poisson()
-        s.run_checkpointed(sched, 3)
-        psiT = s.get_psi()
-        g_lib = s.adjoint(sched, _seed(psiT, rho_t, rbar))
-    assert rel_l2(pT.detach().cpu().numpy(), psiT) < 1e-14
-    assert rel_l2(g_torch, g_lib) < 1e-12
+TORCH_SCRIPT = r"""
+import sys
+sys.path.insert(0, sys.argv[1])
+import torch                      # torch's HIP runtime first: the library then shares it
+torch.cuda.init()
+import numpy as np
+import ifdm_code_amd as F
+from ifdm_code_amd.autograd import evolve
+from tests.test_gpu_adjoint import _setup, _seed
+from tests.fields import rel_l2
+n, steps = 16, 6
+F_, O, rp, cfg, psi0, sched, rho_t, rbar = _setup(n, steps, 0.0, seed=21)
+with F.Solver(cfg) as s:
+    s.set_psi(psi0)
+    s.poisson()
+    p0 = torch.tensor(psi0, dtype=torch.complex128, device="cuda:0", requires_grad=True)
+    pT = evolve(p0, s, sched, 3)
+    rt = torch.tensor(rho_t, device="cuda:0")
+    loss = (((pT.abs() ** 2) - rt) / rbar).pow(2).sum()
+    loss.backward()
+    g_torch = p0.grad.detach().cpu().numpy()
+    s.set_psi(psi0)
+    s.poisson()
+    s.run_checkpointed(sched, 3)
+    psiT = s.get_psi()
+    g_lib = s.adjoint(sched, _seed(psiT, rho_t, rbar))
+e1 = rel_l2(pT.detach().cpu().numpy(), psiT)
+e2 = rel_l2(g_torch, g_lib)
+print("forward", e1, "gradient", e2, flush=True)
+assert e1 < 1e-14 and e2 < 1e-12
+"""
+
+
+def test_torch_autograd_function(tmp_path):
+    """torch.autograd through the library (run in a fresh process that
+    imports torch before the library: both use a HIP runtime with soname
+    libamdhip64.so.7, and the first one loaded serves the process)."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = tmp_path / "torch_check.py"
+    script.write_text(TORCH_SCRIPT)
+    r = subprocess.run([sys.executable, str(script), root], capture_output=True, text=True, timeout=600, cwd=root)
+    print(r.stdout[-2000:], r.stderr[-3000:])
+    assert r.returncode == 0
 
 
 def test_mixed_adjoint_against_fp64():

@@ -125,9 +125,14 @@ def test_snapshot_layout_and_rerun(tmp_path, prec):
         assert np.array_equal(s2.get_psi(), psi) if prec == 0 else rel_l2(s2.get_psi(), psi) < 1e-7
         a3, _ = s2.simulate_native(ar, 3)
         again = s2.get_psi()
-    assert a3 == a2
     if prec == 0:
+        assert a3 == a2
         assert np.array_equal(again, cont)
+    else:
+        # the mixed Poisson source subtracts the FP64 mean density, which the
+        # restart recomputes on the host: FP32-level differences in phi and dt
+        assert a3 == pytest.approx(a2, rel=1e-6)
+        assert rel_l2(again, cont) < 1e-5
 
 
 def test_readsnap_reads_our_snapshot(tmp_path):
