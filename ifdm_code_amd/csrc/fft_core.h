@@ -113,14 +113,17 @@ template <typename T, int N> struct TwBase {
     cx<T> w[4][E];  // [stage][butterfly]; stage 0 unused
 };
 
-template <typename T, int N>
+// tw is the table W_M^k, k < M, of a transform of length M = TS * N
+// (TS = 1: the line's own table; TS = 2: a half-length transform reading the
+// full-length table)
+template <typename T, int N, int TS = 1>
 __device__ __forceinline__ void load_twiddles(TwBase<T, N> &b, int t, const cx<T> *__restrict__ tw) {
     constexpr int E = Sched<N>::E, P = N / E;
 #pragma unroll
     for (int S = 1; S < Sched<N>::ns; ++S) {
         const int R = Sched<N>::r[S], Q = E / R, NS = ns_before<N>(S);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) b.w[S][q] = tw[((t + q * P) & (NS - 1)) * (N / (NS * R))];
+        for (int q = 0; q < Q; ++q) b.w[S][q] = tw[TS * (((t + q * P) & (NS - 1)) * (N / (NS * R)))];
     }
 }
 

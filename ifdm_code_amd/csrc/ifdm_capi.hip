@@ -66,6 +66,26 @@ struct Ops {
         LAUNCHCHK(c);
         return prof_end(c);
     }
+    // pass I over x-planes [x0, x0 + nx): the half-length c2r for N >= 32
+    static int c2r(ifdm_ctx *c, int x0 = 0, int nx = 0) {
+        if constexpr (N < 32) {
+            return zpass<Z_C2R>(c, zargs(c, 0, 0), c->part_phi, x0, nx);
+        } else {
+            constexpr int LZ = c2r_lines<N>();
+            if (nx == 0) nx = N;
+            ZArgs za = zargs(c, 0, 0);
+            const long l0 = (long)x0 * N, nl = (long)nx * N;
+            za.blk0 = (int)(l0 / LZ);
+            const int nb = (int)((nl + LZ - 1) / LZ);
+            int rc;
+            if ((rc = prof_begin(c))) return rc;
+            hipLaunchKernelGGL((k_c2r_half<T, N>), dim3(nb), dim3(z_lines<N>() * P), 0, c->stream,
+                               (const cx<T> *)c->shat, (T *)c->phi, (const cx<T> *)c->tw, za, c->part_phi);
+            LAUNCHCHK(c);
+            return prof_end(c);
+        }
+    }
+
     // strided pass; for AXIS 1 the fixed index (x) runs over [o0, o0 + no)
     template <int AXIS, int MODE>
     static int spass(ifdm_ctx *c, void *arr, int rl, int kv, SArgs sa, int o0 = 0, int no = 0) {
@@ -116,7 +136,7 @@ struct Ops {
             case 5: return spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sa);
             case 6: return spass<0, S_GREEN>(c, c->shat, c->hs, N / 2 + 1, sa);
             case 7: return spass<1, S_INV>(c, c->shat, c->hs, N / 2 + 1, sa);
-            case 8: return zpass<Z_C2R>(c, zargs(c, 0, 0), c->part_phi);
+            case 8: return c2r(c, 0, 0);
             case 9: {
                 int rc;
                 if ((rc = prof_begin(c))) return rc;
@@ -220,7 +240,7 @@ struct Ops {
             c->cur_pass = 7;
             if ((rc = spass<1, S_INV>(c, c->shat, c->hs, N / 2 + 1, sa, x0, nx))) return rc;
             c->cur_pass = 8;
-            if ((rc = zpass<Z_C2R>(c, z0, c->part_phi, x0, nx))) return rc;
+            if ((rc = c2r(c, x0, nx))) return rc;
         }
         return pass(c, 9, 0, 0, 0);
     }

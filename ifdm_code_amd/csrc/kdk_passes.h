@@ -212,6 +212,73 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
     }
 }
 
+// ------------------------------------------------------------- pass I (c2r)
+// Complex-to-real along z with a half-length transform: from the Hermitian
+// half line X[k], k <= N/2 (Re X[0], Re X[N/2] taken), form for k < N/2
+//   Z[k] = (X[k] + conj X[N/2-k]) + i W^-k (X[k] - conj X[N/2-k]),
+// then z = IDFT_{N/2}(Z) (unnormalised) gives y[2m] = Re z[m],
+// y[2m+1] = Im z[m] with y the unnormalised length-N inverse of X.
+// Lines per block are twice those of the full-length z passes.
+template <int N> constexpr int c2r_lines() { return 2 * z_lines<N>(); }
+
+template <typename T, int N>
+__global__ __launch_bounds__(z_lines<N>() * (N / Sched<N>::E))
+void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T> *__restrict__ tw, ZArgs za,
+                double *__restrict__ part) {
+    constexpr int H = N / 2, E = Sched<H>::E, P = H / E, LZ = c2r_lines<N>(), LS = lds_line<H>();
+    constexpr int HS = hs_of(N), NT = LZ * P;
+    static_assert(NT == z_lines<N>() * (N / Sched<N>::E), "block shape");
+    __shared__ cx<T> lds[(LZ * LS > NT / 2 + 1) ? LZ * LS : NT / 2 + 1];
+    const int tid = threadIdx.x, l = tid / P, t = tid - l * P;
+    const int blk = za.blk0 + (int)blockIdx.x;
+    const long line = (long)blk * LZ + l;
+    const bool valid = line < (long)N * N;
+    cx<T> *L = lds + l * LS;
+    TwBase<T, H> tb;
+    load_twiddles<T, H, 2>(tb, t, tw);
+    const cx<T> *row = shat + line * HS;
+    const cx<T> wt = tw[t];                                    // W^t; W^(t+eP) = W^t W^(eP)
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int k = t + e * P;
+        cx<T> a = mkc<T>(0, 0), b = mkc<T>(0, 0);
+        if (valid) {
+            a = row[k];
+            b = row[H - k];
+        }
+        if (k == 0) { a.y = 0; b.y = 0; }                      // Re X[0], Re X[N/2]
+        const cx<T> sum = mkc<T>(a.x + b.x, a.y - b.y);        // X[k] + conj X[N/2-k]
+        cx<T> dif = mkc<T>(a.x - b.x, a.y + b.y);              // X[k] - conj X[N/2-k]
+        cx<T> w = e == 0 ? wt : cmul(wt, tw[e * P]);         // W^k
+        w.y = -w.y;                                            // W^-k
+        dif = cmul(dif, w);
+        v[e] = mkc<T>(sum.x - dif.y, sum.y + dif.x);           // sum + i dif
+    }
+    fft_line<T, H, +1>(v, L, t, tb);
+    double mx = 0.0;
+    cx<T> *out = reinterpret_cast<cx<T> *>(phi + line * N);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int m = t + e * P;
+        if (valid) {
+            out[m] = v[e];                                      // y[2m], y[2m+1]
+            const long f0 = line * (N + 2) + 2 * m;
+            const bool in0 = !za.ref_range || f0 < (long)N * N * N;
+            const bool in1 = !za.ref_range || f0 + 1 < (long)N * N * N;
+            if (in0) mx = fmax(mx, fabs((double)v[e].x));
+            if (in1) mx = fmax(mx, fabs((double)v[e].y));
+        }
+    }
+    double *sb = reinterpret_cast<double *>(lds);
+    const double bm = block_reduce<true>(mx, sb, tid, NT);
+    if (tid == 0) {
+        // partials are indexed per full-length z-pass block: two per block here
+        part[2 * blk] = bm;
+        part[2 * blk + 1] = 0.0;
+    }
+}
+
 // ------------------------------------------------------------ strided passes
 // AXIS 1: lines along y (stride rl); AXIS 0: lines along x (stride N*rl).
 // A work-group takes TC consecutive z-columns so each global access of a
