@@ -26,7 +26,13 @@ def test_single_mode_power(prec):
     assert tot[4] == pytest.approx(L ** 3 * eps ** 2 / 2, rel=1e-10 if prec == 0 else 1e-5)
     others = np.delete(tot, 4)
     assert np.abs(others).max() < (1e-20 if prec == 0 else 1e-9) * tot[4]
-    assert kk[4] == pytest.approx(5 * 2 * np.pi / L, rel=1e-12)
+    # the shell |n| in [4.5, 5.5): its mode-averaged |k|
+    f = np.fft.fftfreq(n) * n
+    nz = np.arange(n // 2 + 1)
+    nn = np.sqrt(f[:, None, None] ** 2 + f[None, :, None] ** 2 + nz[None, None, :] ** 2)
+    w = np.where((nz == 0) | (nz == n // 2), 1.0, 2.0)[None, None, :] * np.ones_like(nn)
+    sel = np.floor(nn + 0.5) == 5
+    assert kk[4] == pytest.approx(2 * np.pi / L * np.sum(w[sel] * nn[sel]) / np.sum(w[sel]), rel=1e-12)
 
 
 def test_power_spectrum_matches_numpy():

@@ -191,8 +191,9 @@ def test_cli_matches_library_and_restarts(tmp_path):
             t, _stop = trig(a, tn, 3, 7)
             if t is not None:
                 raw = np.fromfile(out + f"output{t}.dat", np.float64)
-                assert raw[2] == a
-                assert rel_l2(raw[3 + n:].view(np.complex128).reshape(n, n, n), s.get_psi()) < 1e-13
+                # the two runs deposit the IC with FP64 atomics in different orders
+                assert raw[2] == pytest.approx(a, rel=1e-13)
+                assert rel_l2(raw[3 + n:].view(np.complex128).reshape(n, n, n), s.get_psi()) < 1e-11
             st = s.step(*dt)
     # rerun from output4.dat (ti = 4) for the remaining steps
     out2 = str(tmp_path / "out2") + "/"
