@@ -103,6 +103,60 @@ def timed_adaptive(F, cfg, psi, warmup: int, steps: int, slab: int, prof_steps: 
     return steps / el, el, avg, sched_w + sched_t, psi_out, phi_out, a
 
 
+def _field_errors(s, psi64, phi64, pk64, norm64) -> dict:
+    """Per-field errors of solver s's state against the FP64 run's."""
+    psi = s.get_psi()
+    st = s.poisson()
+    phi = s.get_phi()
+    _k, pk, _nm = s.power_spectrum()
+    rho64, rho = np.abs(psi64) ** 2, np.abs(psi) ** 2
+    w = rho64 / rho64.sum()
+    dphase = np.angle(psi * np.conj(psi64))
+    good = pk64 > 0
+    rel = np.abs(pk[good] / pk64[good] - 1)
+    return {"density_rel_l2": rel_l2(rho, rho64),
+            "phase_rho_weighted_rms_rad": float(np.sqrt(np.sum(w * dphase ** 2))),
+            "potential_rel_l2": rel_l2(phi, phi64),
+            "pk_max_rel": float(rel.max()), "pk_median_rel": float(np.median(rel)),
+            "norm_rel": abs(st["norm"] / norm64 - 1)}
+
+
+def error_study(F, cfg, cfgm, psi, checkpoints, slab: int) -> list:
+    """Precision error against the FP64 GPU run of the same IC.  The FP64
+    run takes adaptive steps (ifdm_simulate); the mixed path replays its
+    schedule, and so does an FP64 run started from the IC rounded to
+    complex64 -- the error the FP32 rounding of the IC alone grows into,
+    i.e. the floor set by the dynamics.  At each checkpoint (cumulative step
+    counts) the per-field errors of both against the FP64 run are
+    recorded."""
+    s64, smx, sfl = F.Solver(cfg), F.Solver(cfgm), F.Solver(cfg)
+    try:
+        for s, p in ((s64, psi), (smx, psi), (sfl, psi.astype(np.complex64).astype(np.complex128))):
+            s.set_option("slab_planes", slab)
+            s.set_psi(p)
+            s.poisson()
+        a, done, rows = 1.0 / 51, 0, []
+        for c in checkpoints:
+            a, sch = s64.simulate_native(a, c - done)
+            smx.run(sch)
+            sfl.run(sch)
+            done = c
+            psi64 = s64.get_psi()
+            norm64 = s64.poisson()["norm"]
+            phi64 = s64.get_phi()
+            _k, pk64, _nm = s64.power_spectrum()
+            row = {"steps": c, "a": a,
+                   "mixed": _field_errors(smx, psi64, phi64, pk64, norm64),
+                   "fp64_from_c64_rounded_ic": _field_errors(sfl, psi64, phi64, pk64, norm64)}
+            log(f"[bench] error after {c} steps (a = {a:.5f}): {json.dumps(row)}")
+            rows.append(row)
+            del psi64, phi64
+        return rows
+    finally:
+        for s in (s64, smx, sfl):
+            s.close()
+
+
 def counter_traffic(n: int, precision: str) -> dict:
     """HBM bytes per dispatch of each pass from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (profiles/kdk_<n>_<precision>_hbm_traffic.md),
@@ -140,6 +194,9 @@ def main():
     ap.add_argument("--slab", type=int, default=-1, help="x-planes per slab chain (-1: library default)")
     ap.add_argument("--prof-steps", type=int, default=5)
     ap.add_argument("--sweep-slab", type=str, default="", help="comma list of slab sizes to time (FP64)")
+    ap.add_argument("--error-steps", type=str, default="1000",
+                    help="comma list of cumulative step counts at which the mixed path's error against the "
+                         "FP64 run is reported ('' to skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -232,6 +289,9 @@ def main():
             "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avgm)},
         }
         del psi64, psim, rho64, rhom, w, dphase, phim, phi64
+        cps = [int(x) for x in args.error_steps.split(",") if x.strip()]
+        if cps:
+            out["mixed"]["error_vs_fp64"] = error_study(F, cfg, cfgm, psi, sorted(cps), slab)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(n, psi, g_in, xmax_in, args.cpu_steps)
