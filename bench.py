@@ -103,6 +103,17 @@ def timed_adaptive(F, cfg, psi, warmup: int, steps: int, slab: int, prof_steps: 
     return steps / el, el, avg, sched_w + sched_t, psi_out, phi_out, a
 
 
+def _phase_errors(psi, psi64, w):
+    """(rms, global): the density-weighted rms of the wrapped phase
+    difference after removing its weighted circular mean (a global phase,
+    which no observable depends on), and that mean."""
+    d = psi * np.conj(psi64)
+    z = np.sum(w * d / np.maximum(np.abs(d), 1e-300))
+    g = float(np.angle(z))
+    dp = np.angle(d * complex(np.cos(-g), np.sin(-g)))
+    return float(np.sqrt(np.sum(w * dp * dp))), g
+
+
 def _field_errors(s, psi64, phi64, pk64, norm64) -> dict:
     """Per-field errors of solver s's state against the FP64 run's."""
     psi = s.get_psi()
@@ -111,47 +122,68 @@ def _field_errors(s, psi64, phi64, pk64, norm64) -> dict:
     _k, pk, _nm = s.power_spectrum()
     rho64, rho = np.abs(psi64) ** 2, np.abs(psi) ** 2
     w = rho64 / rho64.sum()
-    dphase = np.angle(psi * np.conj(psi64))
+    ph, gph = _phase_errors(psi, psi64, w)
     good = pk64 > 0
-    rel = np.abs(pk[good] / pk64[good] - 1)
+    rel = np.abs(pk[good] / pk64[good] - 1) if good.any() else np.array([np.nan])
     return {"density_rel_l2": rel_l2(rho, rho64),
-            "phase_rho_weighted_rms_rad": float(np.sqrt(np.sum(w * dphase ** 2))),
+            "phase_rho_weighted_rms_rad": ph, "global_phase_rad": gph,
             "potential_rel_l2": rel_l2(phi, phi64),
             "pk_max_rel": float(rel.max()), "pk_median_rel": float(np.median(rel)),
             "norm_rel": abs(st["norm"] / norm64 - 1)}
 
 
-def error_study(F, cfg, cfgm, psi, checkpoints, slab: int) -> list:
-    """Precision error against the FP64 GPU run of the same IC.  The FP64
-    run takes adaptive steps (ifdm_simulate); the mixed path replays its
-    schedule, and so does an FP64 run started from the IC rounded to
-    complex64 -- the error the FP32 rounding of the IC alone grows into,
-    i.e. the floor set by the dynamics.  At each checkpoint (cumulative step
-    counts) the per-field errors of both against the FP64 run are
-    recorded."""
+def round_c64(psi: np.ndarray) -> np.ndarray:
+    """psi rounded to complex64 precision (through an exact power-of-two
+    scaling, since |psi| exceeds the FP32 range)."""
+    e = int(np.frexp(np.max(np.abs(psi)))[1])
+    f = 2.0 ** -e
+    return (psi * f).astype(np.complex64).astype(np.complex128) / f
+
+
+def error_study(F, cfg, cfgm, psi, checkpoints, slab: int, a_stop: float = 0.5) -> dict:
+    """Precision error against the FP64 GPU run of the same IC.
+
+    Schedule: the FP64 adaptive run (Timestep_criteria each step) up to the
+    reference's own end of run, the first step that takes a past 0.5
+    (src/CosmoBEC_MPI_stan.cpp:250-255).  When the largest checkpoint asks
+    for more steps than that, every step is split into m equal parts
+    (D_KE, D_PE, D_QI each divided by m) so the same expansion span is
+    covered in that many steps.  The FP64 path, the mixed path and an FP64
+    run started from the IC rounded to complex64 (the error the rounding of
+    the IC alone grows into: the floor the dynamics set) replay that
+    schedule; at each checkpoint (cumulative step counts) the per-field
+    errors of the latter two against the FP64 path are recorded."""
+    sched, a = [], 1.0 / 51
+    with F.Solver(cfg) as sg:
+        sg.set_psi(psi)
+        while a <= a_stop and len(sched) < max(checkpoints):
+            a, d = sg.simulate_native(a, 1)
+            sched += d
+    m = max(1, -(-max(checkpoints) // len(sched)))
+    steps = [tuple(x / m for x in d) for d in sched for _ in range(m)]
+    cps = sorted({min(c, len(steps)) for c in checkpoints})
+    log(f"[bench] error study: {len(sched)} adaptive steps to a = {a:.4f}, split {m}x -> {len(steps)} steps")
     s64, smx, sfl = F.Solver(cfg), F.Solver(cfgm), F.Solver(cfg)
     try:
-        for s, p in ((s64, psi), (smx, psi), (sfl, psi.astype(np.complex64).astype(np.complex128))):
+        for s, p in ((s64, psi), (smx, psi), (sfl, round_c64(psi))):
             s.set_option("slab_planes", slab)
             s.set_psi(p)
             s.poisson()
-        a, done, rows = 1.0 / 51, 0, []
-        for c in checkpoints:
-            a, sch = s64.simulate_native(a, c - done)
-            smx.run(sch)
-            sfl.run(sch)
+        done, rows = 0, []
+        for c in cps:
+            for s in (s64, smx, sfl):
+                s.run(steps[done:c])
             done = c
             psi64 = s64.get_psi()
             norm64 = s64.poisson()["norm"]
             phi64 = s64.get_phi()
             _k, pk64, _nm = s64.power_spectrum()
-            row = {"steps": c, "a": a,
-                   "mixed": _field_errors(smx, psi64, phi64, pk64, norm64),
+            row = {"steps": c, "mixed": _field_errors(smx, psi64, phi64, pk64, norm64),
                    "fp64_from_c64_rounded_ic": _field_errors(sfl, psi64, phi64, pk64, norm64)}
-            log(f"[bench] error after {c} steps (a = {a:.5f}): {json.dumps(row)}")
+            log(f"[bench] error after {c} steps: {json.dumps(row)}")
             rows.append(row)
             del psi64, phi64
-        return rows
+        return {"adaptive_steps": len(sched), "a_end": a, "split": m, "steps": len(steps), "checkpoints": rows}
     finally:
         for s in (s64, smx, sfl):
             s.close()
@@ -194,7 +226,7 @@ def main():
     ap.add_argument("--slab", type=int, default=-1, help="x-planes per slab chain (-1: library default)")
     ap.add_argument("--prof-steps", type=int, default=5)
     ap.add_argument("--sweep-slab", type=str, default="", help="comma list of slab sizes to time (FP64)")
-    ap.add_argument("--error-steps", type=str, default="1000",
+    ap.add_argument("--error-steps", type=str, default="100,1000",
                     help="comma list of cumulative step counts at which the mixed path's error against the "
                          "FP64 run is reported ('' to skip)")
     args = ap.parse_args()
@@ -275,7 +307,7 @@ def main():
             phim = sr.get_phi()
         rho64, rhom = np.abs(psi64) ** 2, np.abs(psim) ** 2
         w = rho64 / rho64.sum()
-        dphase = np.angle(psim * np.conj(psi64))
+        ph, _g = _phase_errors(psim, psi64, w)
         pbm = pass_bytes(n, 8, 4)
         out["mixed"] = {
             "value": round(spsm, 3), "unit": "KDK steps/s", "ms_per_step": round(1e3 * elm / args.steps, 4),
@@ -284,11 +316,11 @@ def main():
             "step_hbm_frac_of_8TBs": round(sum(pbm) * spsm / 1e9 / HBM_PEAK_GBS, 4),
             "error_vs_fp64_after_steps": {
                 "steps": len(sched), "density_rel_l2": rel_l2(rhom, rho64),
-                "phase_rho_weighted_rms_rad": float(np.sqrt(np.sum(w * dphase ** 2))),
+                "phase_rho_weighted_rms_rad": ph,
                 "potential_rel_l2": rel_l2(phim, phi64)},
             "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avgm)},
         }
-        del psi64, psim, rho64, rhom, w, dphase, phim, phi64
+        del psi64, psim, rho64, rhom, w, phim, phi64
         cps = [int(x) for x in args.error_steps.split(",") if x.strip()]
         if cps:
             out["mixed"]["error_vs_fp64"] = error_study(F, cfg, cfgm, psi, sorted(cps), slab)

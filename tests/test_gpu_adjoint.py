@@ -162,8 +162,15 @@ def test_torch_autograd_function(tmp_path):
 def test_mixed_adjoint_against_fp64():
     """The mixed-precision adjoint (FP32 fields, power-of-two scaled psi and
     g, FP64 phases) against the FP64 adjoint on the same schedule, for a
-    cold field (density rms 1%, phase rms 0.05 rad): the gradients agree to
-    FP32 storage accuracy amplified over the steps."""
+    cold field (density rms 1%, phase rms 0.05 rad).
+
+    * Same seed (made from the FP64 forward state) in both sweeps: the
+      reverse sweep's own precision, to FP32 accuracy amplified over the
+      steps.
+    * Each path's own seed: the seed 4(rho_T - rho_tgt) psi_T differences
+      two nearly equal densities, so the mixed forward's error enters it
+      amplified by about rho / |rho_T - rho_tgt|; the gradient may not
+      amplify that seed error further (factor <= 10)."""
     import ifdm_code_amd as F
     from oracle import oracle as O
     n, steps = 32, 10
@@ -175,15 +182,23 @@ def test_mixed_adjoint_against_fp64():
     tgt, _ = O.simulate(rp, make_psi(n, 18, cfg64.mfdm_solar, delta_rms=0.01, phase_rms=0.05), 1.0 / 51, steps)
     rho_t = np.abs(tgt.psi) ** 2
     rbar = float(np.mean(np.abs(psi0) ** 2))
-    grads = []
-    for prec in (0, 1):
+
+    def grad(prec, seed_fn):
         c = F.Config.from_cli(2.5e-24, 0.0, 5.0, n, precision=prec)
         with F.Solver(c) as s:
             s.set_psi(psi0)
             s.poisson()
             s.run_checkpointed(sched, 5)
             psiT = s.get_psi()
-            grads.append(s.adjoint(sched, _seed(psiT, rho_t, rbar)))
-    err = rel_l2(grads[1], grads[0])
-    _say(f"   mixed vs FP64 adjoint gradient rel L2 = {err:.3e}")
-    assert err < 1e-4
+            seed = seed_fn(psiT)
+            return s.adjoint(sched, seed), seed
+
+    g64, seed64 = grad(0, lambda p: _seed(p, rho_t, rbar))
+    g_same, _ = grad(1, lambda p: seed64)
+    g_own, seed_mx = grad(1, lambda p: _seed(p, rho_t, rbar))
+    err_sweep = rel_l2(g_same, g64)
+    err_seed = rel_l2(seed_mx, seed64)
+    err_full = rel_l2(g_own, g64)
+    _say(f"   mixed adjoint: same seed {err_sweep:.3e}; seed error {err_seed:.3e}; own seed {err_full:.3e}")
+    assert err_sweep < 1e-4
+    assert err_full < 10 * err_seed + 1e-4
