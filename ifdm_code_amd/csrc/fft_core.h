@@ -145,10 +145,27 @@ __device__ __forceinline__ void apply_twiddles(cx<T> *a, cx<T> w1) {
     }
 }
 
+// Ordering of the LDS exchanges of one line.  WAVE: every point of the line
+// is held by one wavefront, whose LDS operations are performed in issue
+// order, so only the compiler must be kept from moving LDS accesses across
+// the exchange (no s_barrier; the wavefronts of a work-group run
+// independently).  Otherwise the line spans wavefronts and the work-group
+// synchronises.
+template <bool WAVE> __device__ __forceinline__ void line_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // One Stockham stage S of an N-point transform, direction DIR (-1 forward,
 // +1 inverse), for the thread at line position t.  L is the line's LDS
-// region; b the thread's base twiddles.
-template <typename T, int N, int DIR, int S>
+// region; b the thread's base twiddles.  FRESH: the work-group has not yet
+// used its LDS, so the first scatter needs no preceding synchronisation.
+template <typename T, int N, int DIR, int S, bool WAVE = false, bool FRESH = false>
 __device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &b) {
     constexpr int E = Sched<N>::E, P = N / E, R = Sched<N>::r[S], Q = E / R;
     constexpr int NS = ns_before<N>(S);
@@ -171,7 +188,7 @@ __device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const TwBas
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q + r * Q] = a[q][r];
     } else {
-        __syncthreads();
+        if constexpr (!(FRESH && FIRST)) line_sync<WAVE>();
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int j = t + q * P;
@@ -179,24 +196,28 @@ __device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const TwBas
 #pragma unroll
             for (int r = 0; r < R; ++r) L[lpos(base + r * NS)] = a[q][r];
         }
-        __syncthreads();
+        line_sync<WAVE>();
     }
 }
 
-template <typename T, int N, int DIR>
+template <typename T, int N, int DIR, bool WAVE = false, bool FRESH = false>
 __device__ __forceinline__ void fft_line(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &b) {
-    fft_stage<T, N, DIR, 0>(v, L, t, b);
-    if constexpr (Sched<N>::ns > 1) fft_stage<T, N, DIR, 1>(v, L, t, b);
-    if constexpr (Sched<N>::ns > 2) fft_stage<T, N, DIR, 2>(v, L, t, b);
-    if constexpr (Sched<N>::ns > 3) fft_stage<T, N, DIR, 3>(v, L, t, b);
+    fft_stage<T, N, DIR, 0, WAVE, FRESH>(v, L, t, b);
+    if constexpr (Sched<N>::ns > 1) fft_stage<T, N, DIR, 1, WAVE>(v, L, t, b);
+    if constexpr (Sched<N>::ns > 2) fft_stage<T, N, DIR, 2, WAVE>(v, L, t, b);
+    if constexpr (Sched<N>::ns > 3) fft_stage<T, N, DIR, 3, WAVE>(v, L, t, b);
 }
 
 // convenience: load the base twiddles and transform
-template <typename T, int N, int DIR>
+template <typename T, int N, int DIR, bool WAVE = false>
 __device__ __forceinline__ void fft_line(cx<T> *v, cx<T> *L, int t, const cx<T> *__restrict__ tw) {
     TwBase<T, N> b;
     load_twiddles<T, N>(b, t, tw);
-    fft_line<T, N, DIR>(v, L, t, b);
+    fft_line<T, N, DIR, WAVE>(v, L, t, b);
 }
+
+// true when a line of N points (N/E consecutive threads) lies inside one
+// wavefront of 64 lanes
+template <int N> constexpr bool wave_line() { return N / Sched<N>::E <= 64; }
 
 }  // namespace ifdm

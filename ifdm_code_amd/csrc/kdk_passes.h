@@ -124,7 +124,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             }
             v[e] = s;
         }
-        fft_line<T, N, +1>(v, L, t, tb);
+        fft_line<T, N, +1, wave_line<N>()>(v, L, t, tb);
         double mx = 0.0;
         T *out = phi + line * N;
 #pragma unroll
@@ -146,7 +146,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         cx<T> s[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) s[e] = mkc<T>(valid ? (T)(za.ms * (double)rrow[t + e * P]) : (T)0, (T)0);
-        fft_line<T, N, -1>(s, L, t, tb);
+        fft_line<T, N, -1, wave_line<N>()>(s, L, t, tb);
         if (valid) {
             cx<T> *srow = shat + line * HS;
 #pragma unroll
@@ -172,7 +172,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         }
         if constexpr (MODE == Z_FWD || MODE == Z_KICK_FWD) {
             if (za.dtab && blk == 0) write_drift_table<T, N>((cx<T> *)za.dtab, za.k2, za.ke_dke, tid, NT);
-            fft_line<T, N, -1>(v, L, t, tb);
+            fft_line<T, N, -1, wave_line<N>()>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -180,7 +180,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             return;
         }
         if constexpr (MODE == Z_INV || MODE == Z_INV_SRC) {
-            fft_line<T, N, +1>(v, L, t, tb);
+            fft_line<T, N, +1, wave_line<N>()>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -195,7 +195,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
                 if (valid) { mx = fmax(mx, rr); sm += rr; }
                 s[e] = mkc<T>(valid ? (T)(za.ms * rr - za.rhoc) : (T)0, (T)0);
             }
-            fft_line<T, N, -1>(s, L, t, tb);
+            fft_line<T, N, -1, wave_line<N>()>(s, L, t, tb);
             if (valid) {
                 cx<T> *srow = shat + line * HS;
 #pragma unroll
@@ -255,7 +255,7 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
         dif = cmul(dif, w);
         v[e] = mkc<T>(sum.x - dif.y, sum.y + dif.x);           // sum + i dif
     }
-    fft_line<T, H, +1>(v, L, t, tb);
+    fft_line<T, H, +1, wave_line<H>()>(v, L, t, tb);
     double mx = 0.0;
     cx<T> *out = reinterpret_cast<cx<T> *>(phi + line * N);
 #pragma unroll
@@ -284,15 +284,16 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
 // A work-group takes TC consecutive z-columns so each global access of a
 // wavefront covers 128-byte row segments.
 
-// the transform (and the spectral multiply of passes C and G) of one tile
-template <typename T, int N, int MODE>
+// the transform (and the spectral multiply of passes C and G) of one tile;
+// FRESH: the first transform of the work-group (no earlier LDS use)
+template <typename T, int N, int MODE, bool FRESH>
 __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &tw,
                                               const SArgs &sa, int other, int k, bool valid) {
     constexpr int E = Sched<N>::E, P = N / E;
-    if constexpr (MODE == S_FWD) fft_line<T, N, -1>(v, L, t, tw);
-    else if constexpr (MODE == S_INV) fft_line<T, N, +1>(v, L, t, tw);
+    if constexpr (MODE == S_FWD) fft_line<T, N, -1, false, FRESH>(v, L, t, tw);
+    else if constexpr (MODE == S_INV) fft_line<T, N, +1, false, FRESH>(v, L, t, tw);
     else {
-        fft_line<T, N, -1>(v, L, t, tw);
+        fft_line<T, N, -1, false, FRESH>(v, L, t, tw);
         // spectral indices of this thread's points: along the line i, fixed `other` and k
         if constexpr (MODE == S_DRIFT) {
             const cx<T> *tab = (const cx<T> *)sa.dtab;
@@ -360,7 +361,7 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = valid ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
-    spass_compute<T, N, MODE>(v, L, t, tb, sa, other, k, valid);
+    spass_compute<T, N, MODE, true>(v, L, t, tb, sa, other, k, valid);
     if (valid) {
 #pragma unroll
         for (int e = 0; e < E; ++e) io.store(c + (long)(t + e * P) * stride, v[e]);
@@ -409,7 +410,7 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__
         }
         const int other = sa.other0 + tile / ntz, k0 = (tile % ntz) * TC, k = k0 + c;
         const bool valid = k < kv;
-        spass_compute<T, N, MODE>(v, L, t, tb, sa, other, k, valid);
+        spass_compute<T, N, MODE, false>(v, L, t, tb, sa, other, k, valid);
         {
             long b0, st;
             spass_addr<N, AXIS>(other, k0, rl, b0, st);
