@@ -32,3 +32,18 @@ def make_psi(n: int, seed: int, mfdm_solar: float, rhoc: float = 3.51059e10, del
 
 def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def snapshot_bytes(psi: np.ndarray, xmax: float, a: float, mfdm_ev: float, g_ev_cm3: float) -> bytes:
+    """A snapshot in the reference's layout (src/write3D.h:86-97,
+    src/readsnap.cpp:48-52): f64 mfdm [eV], f64 g [eV cm^3], f64 a,
+    f64 x[N] with x_i = xmax/N - xmax + i dx (src/struct3D.h:72), then psi
+    as N^3 complex128 in C order."""
+    n = psi.shape[0]
+    x = np.array([xmax / n - xmax + i * (2.0 * xmax / n) for i in range(n)], "<f8")
+    head = np.array([mfdm_ev, g_ev_cm3, a], "<f8")
+    return head.tobytes() + x.tobytes() + np.ascontiguousarray(psi, "<c16").tobytes()
+
+
+# inputs of the golden snapshot fixture (tests/golden/readsnap_n8_z3.txt)
+GOLDEN_SNAP = {"n": 8, "seed": 3, "xmax_mpc_h": 5.0, "a": 0.1, "mfdm_ev": 2.5e-24, "g_ev_cm3": 1e-60, "zslice": 3}

@@ -1,0 +1,73 @@
+"""Writes the golden fixtures in this folder.  Run from the repository root
+in the build container, where oracle/_ref/readsnap has been built from the
+reference's own source (src/readsnap.cpp):
+
+    python tests/golden/make_golden.py
+
+readsnap_n8_z3.txt   the reference's snapshot reader applied to a snapshot of
+                     a seeded psi (n = 8) written in the write3D.h layout
+                     (tests/fields.snapshot_bytes); its ASCII z-slice.
+scale_integrals.json the time integrals D_n = int da / (H(a) a^n), n = 2, 3, 4,
+                     by mpmath at 40 digits (src/struct3D.h:127-146).
+kdk_16_oracle.npz    the oracle's FP64 state after 3 adaptive KDK steps from a
+                     seeded 16^3 psi (a regression vector of the oracle, which
+                     the closed-form tests pin).
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O          # noqa: E402
+from tests.fields import GOLDEN_SNAP, make_psi, snapshot_bytes   # noqa: E402
+
+INTEGRAL_CASES = [(1 / 51, 1 / 51 * 1.002), (0.02, 0.031), (0.02, 1.0), (0.3, 0.5), (0.05, 0.0500001),
+                  (0.1, 0.2), (0.5, 0.9)]
+
+
+def readsnap_fixture():
+    g = GOLDEN_SNAP
+    exe = os.path.join(ROOT, "oracle", "_ref", "readsnap")
+    rp = O.RunParams(n=g["n"], mfdm=g["mfdm_ev"], g_in=g["g_ev_cm3"], xmax_in=g["xmax_mpc_h"])
+    psi = make_psi(g["n"], g["seed"], rp.mfdm_solar)
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "output1.dat"), "wb") as f:
+            f.write(snapshot_bytes(psi, rp.xmax, g["a"], g["mfdm_ev"], g["g_ev_cm3"]))
+        subprocess.run([exe, str(g["n"]), str(g["zslice"]), "1", d, d], check=True, capture_output=True)
+        txt = open(os.path.join(d, f"output1.dat.z{g['zslice']}")).read()
+    open(os.path.join(HERE, "readsnap_n8_z3.txt"), "w").write(txt)
+
+
+def integral_fixture():
+    import mpmath as mp
+    mp.mp.dps = 40
+    H0, Om = mp.mpf(O.H_0), mp.mpf(O.Omega_m0)
+    rows = []
+    for power in (2, 3, 4):
+        for a0, a1 in INTEGRAL_CASES:
+            v = mp.quad(lambda x: 1 / (H0 * mp.sqrt(Om / x ** 3 + (1 - Om)) * x ** power), [a0, a1])
+            rows.append({"power": power, "a0": a0, "a1": a1, "value": mp.nstr(v, 25)})
+    json.dump({"H_0": O.H_0, "Omega_m0": O.Omega_m0, "cases": rows},
+              open(os.path.join(HERE, "scale_integrals.json"), "w"), indent=1)
+
+
+def kdk_fixture():
+    n = 16
+    rp = O.RunParams(n=n)
+    psi0 = make_psi(n, 5, rp.mfdm_solar)
+    st, used = O.simulate(rp, psi0, 1.0 / 51, 3)
+    np.savez_compressed(os.path.join(HERE, "kdk_16_oracle.npz"), psi=st.psi,
+                        sched=np.array([u[:3] for u in used]), a=np.array([u[3] for u in used]))
+
+
+if __name__ == "__main__":
+    readsnap_fixture()
+    integral_fixture()
+    kdk_fixture()

@@ -202,3 +202,25 @@ def test_cli_matches_library_and_restarts(tmp_path):
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert os.path.exists(out2 + "output7.dat")
+
+
+def test_snapshot_bytes_match_the_golden_layout(tmp_path):
+    """ifdm_write_snapshot writes exactly the bytes of the layout that the
+    reference's reader was shown to read (tests/golden/readsnap_n8_z3.txt),
+    and ifdm_read_snapshot restores psi bit for bit."""
+    from tests.fields import GOLDEN_SNAP, make_psi, snapshot_bytes
+    F, _icgen, O = _mods()
+    g = GOLDEN_SNAP
+    _say(f"[snapshot golden] n={g['n']}")
+    cfg = F.Config.from_cli(g["mfdm_ev"], g["g_ev_cm3"], g["xmax_mpc_h"], g["n"])
+    psi = make_psi(g["n"], g["seed"], cfg.mfdm_solar)
+    path = str(tmp_path / "output1.dat")
+    with F.Solver(cfg) as s:
+        s.set_psi(psi)
+        s.write_snapshot(path, g["a"], g["mfdm_ev"], g["g_ev_cm3"])
+        s.set_psi(np.zeros_like(psi))
+        a, m, gg = s.read_snapshot(path)
+        back = s.get_psi()
+    assert open(path, "rb").read() == snapshot_bytes(psi, cfg.xmax, g["a"], g["mfdm_ev"], g["g_ev_cm3"])
+    assert (a, m, gg) == (g["a"], g["mfdm_ev"], g["g_ev_cm3"])
+    assert np.array_equal(back, psi)

@@ -275,3 +275,21 @@ def test_slab_schedule_is_identical(n, slab, prec):
     assert outs[0][2] == outs[1][2]
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_golden_oracle_trajectory():
+    """Three adaptive steps from the seeded 16^3 field of the committed
+    oracle fixture (tests/golden/kdk_16_oracle.npz): same schedule and state
+    without running the oracle."""
+    import os
+    F, _O = _mods()
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "kdk_16_oracle.npz"))
+    cfg = _cfg(F, 16)
+    _say("[golden] n=16 3 adaptive steps")
+    with F.Solver(cfg) as s:
+        s.set_psi(make_psi(16, 5, cfg.mfdm_solar))
+        a, sched = s.simulate_native(1.0 / 51, 3)
+        out = s.get_psi()
+    assert np.allclose(np.array(sched), fx["sched"], rtol=1e-12, atol=0)
+    assert a == pytest.approx(fx["a"][-1], rel=1e-14)
+    assert rel_l2(out, fx["psi"]) < 1e-10
