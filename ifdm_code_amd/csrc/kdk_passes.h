@@ -27,8 +27,10 @@
 
 namespace ifdm {
 
-// half-spectrum row length
-__host__ __device__ constexpr int hs_of(int n) { return ((n / 2 + 1 + 7) / 8) * 8; }
+// half-spectrum row length: N/2+1 rounded up to 16 elements, so every row
+// starts on a 128-byte boundary in both precisions (8 complex doubles or 16
+// complex floats per 128 bytes) and a column tile never straddles lines
+__host__ __device__ constexpr int hs_of(int n) { return ((n / 2 + 1 + 15) / 16) * 16; }
 
 struct ZArgs {
     double aphi;      // kick: phase coefficient of phi

@@ -12,6 +12,13 @@ scale_integrals.json the time integrals D_n = int da / (H(a) a^n), n = 2, 3, 4,
 kdk_16_oracle.npz    the oracle's FP64 state after 3 adaptive KDK steps from a
                      seeded 16^3 psi (a regression vector of the oracle, which
                      the closed-form tests pin).
+long_128_oracle.npz  the 128^3 FP64 run of 1000 steps from the Zel'dovich
+                     particle IC (box 10 Mpc/h, z = 50, seed 1234, CIC into
+                     psi): the schedule (adaptive Timestep_criteria steps up
+                     to the reference's end of run at a > 0.5, each split into
+                     equal parts to make 1000 steps) and psi at steps 0, 100
+                     and 1000 on 4096 seeded sample cells, with norm and
+                     maxima.  Takes a few minutes on 8 cores.
 """
 import json
 import os
@@ -67,7 +74,60 @@ def kdk_fixture():
                         sched=np.array([u[:3] for u in used]), a=np.array([u[3] for u in used]))
 
 
+LONG = {"n": 128, "xmax_mpc_h": 5.0, "seed": 1234, "steps": 1000, "checks": (100, 1000), "samples": 4096,
+        "a_stop": 0.5}
+
+
+def long_schedule(rp, psi0, a_stop, total):
+    """Adaptive steps while a <= a_stop (src/CosmoBEC_MPI_stan.cpp:250-255
+    ends the run at the first a past 0.5), each split into m equal parts
+    (D_KE, D_PE, D_QI divided by m), the first `total` of them."""
+    st = O.State(rp, psi0)
+    a, sched = 1.0 / 51, []
+    while a <= a_stop:
+        st.poisson()
+        m = st.maxima()
+        dKE, dPE, dQI, a, _ = O.timestep_criteria(rp, a, m["max_phi"], m["max_rho"])
+        st.split_op(dKE, dPE, dQI)
+        sched.append((dKE, dPE, dQI))
+    m = -(-total // len(sched))
+    return [tuple(x / m for x in d) for d in sched for _ in range(m)][:total], len(sched), a
+
+
+def long_fixture():
+    from ifdm_code_amd import icgen
+    L = LONG
+    n = L["n"]
+    rp = O.RunParams(n=n, xmax_in=L["xmax_mpc_h"])
+    pos, vel, pm, a0 = icgen.zeldovich(n, 2 * L["xmax_mpc_h"], z=50.0, seed=L["seed"])
+    psi0, _th, _sk = O.ic_from_particles(rp, [(pos, vel)], 2 * L["xmax_mpc_h"] * 1e3, pm * 1e10, a0)
+    steps, nad, a_end = long_schedule(rp, psi0, L["a_stop"], L["steps"])
+    idx = np.random.default_rng(7).choice(n ** 3, L["samples"], replace=False)
+    out = {"sched": np.array(steps), "adaptive_steps": nad, "a_end": a_end, "idx": idx, "a0": a0,
+           "psi0": psi0.ravel()[idx], "norm0": float(np.sum(np.abs(psi0) ** 2))}
+    st = O.State(rp, psi0)
+    done = 0
+    for c in L["checks"]:
+        for d in steps[done:c]:
+            st.poisson()
+            st.split_op(*d)
+        done = c
+        st.poisson()
+        m = st.maxima()
+        out[f"psi{c}"] = st.psi.ravel()[idx]
+        out[f"norm{c}"] = float(np.sum(np.abs(st.psi) ** 2))
+        out[f"maxphi{c}"] = m["max_phi"]
+        out[f"maxrho{c}"] = m["max_rho"]
+    np.savez_compressed(os.path.join(HERE, "long_128_oracle.npz"), **out)
+
+
 if __name__ == "__main__":
-    readsnap_fixture()
-    integral_fixture()
-    kdk_fixture()
+    which = sys.argv[1:] or ["readsnap", "integrals", "kdk", "long"]
+    if "readsnap" in which:
+        readsnap_fixture()
+    if "integrals" in which:
+        integral_fixture()
+    if "kdk" in which:
+        kdk_fixture()
+    if "long" in which:
+        long_fixture()

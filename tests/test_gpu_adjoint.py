@@ -164,13 +164,14 @@ def test_mixed_adjoint_against_fp64():
     g, FP64 phases) against the FP64 adjoint on the same schedule, for a
     cold field (density rms 1%, phase rms 0.05 rad).
 
-    * Same seed (made from the FP64 forward state) in both sweeps: the
-      reverse sweep's own precision, to FP32 accuracy amplified over the
-      steps.
-    * Each path's own seed: the seed 4(rho_T - rho_tgt) psi_T differences
-      two nearly equal densities, so the mixed forward's error enters it
-      amplified by about rho / |rho_T - rho_tgt|; the gradient may not
-      amplify that seed error further (factor <= 10)."""
+    * A linear loss L = Re sum conj(c) psi_T (seed c, fixed): both sweeps
+      get the same seed, and differ by the FP32 round-off of the mixed
+      forward trajectory and reverse sweep (bound 1e-3).
+    * The density loss with each path's own seed: the seed
+      4(rho_T - rho_tgt) psi_T differences two nearly equal densities, so the
+      mixed forward's error enters it amplified by about
+      rho / |rho_T - rho_tgt|; the gradient may not amplify that seed error
+      further (factor <= 10)."""
     import ifdm_code_amd as F
     from oracle import oracle as O
     n, steps = 32, 10
@@ -182,6 +183,8 @@ def test_mixed_adjoint_against_fp64():
     tgt, _ = O.simulate(rp, make_psi(n, 18, cfg64.mfdm_solar, delta_rms=0.01, phase_rms=0.05), 1.0 / 51, steps)
     rho_t = np.abs(tgt.psi) ** 2
     rbar = float(np.mean(np.abs(psi0) ** 2))
+    rng = np.random.default_rng(11)
+    c_lin = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
 
     def grad(prec, seed_fn):
         c = F.Config.from_cli(2.5e-24, 0.0, 5.0, n, precision=prec)
@@ -189,16 +192,17 @@ def test_mixed_adjoint_against_fp64():
             s.set_psi(psi0)
             s.poisson()
             s.run_checkpointed(sched, 5)
-            psiT = s.get_psi()
-            seed = seed_fn(psiT)
+            seed = seed_fn(s.get_psi())
             return s.adjoint(sched, seed), seed
 
+    g64_lin, _ = grad(0, lambda p: c_lin)
+    gmx_lin, _ = grad(1, lambda p: c_lin)
     g64, seed64 = grad(0, lambda p: _seed(p, rho_t, rbar))
-    g_same, _ = grad(1, lambda p: seed64)
-    g_own, seed_mx = grad(1, lambda p: _seed(p, rho_t, rbar))
-    err_sweep = rel_l2(g_same, g64)
+    gmx, seed_mx = grad(1, lambda p: _seed(p, rho_t, rbar))
+    err_lin = rel_l2(gmx_lin, g64_lin)
     err_seed = rel_l2(seed_mx, seed64)
-    err_full = rel_l2(g_own, g64)
-    _say(f"   mixed adjoint: same seed {err_sweep:.3e}; seed error {err_seed:.3e}; own seed {err_full:.3e}")
-    assert err_sweep < 1e-4
+    err_full = rel_l2(gmx, g64)
+    _say(f"   mixed adjoint: linear loss {err_lin:.3e}; density loss seed error {err_seed:.3e}, "
+         f"gradient error {err_full:.3e}")
+    assert err_lin < 1e-3
     assert err_full < 10 * err_seed + 1e-4

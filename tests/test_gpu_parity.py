@@ -293,3 +293,45 @@ def test_golden_oracle_trajectory():
     assert np.allclose(np.array(sched), fx["sched"], rtol=1e-12, atol=0)
     assert a == pytest.approx(fx["a"][-1], rel=1e-14)
     assert rel_l2(out, fx["psi"]) < 1e-10
+
+
+def test_long_run_against_oracle_fixture():
+    """128^3 FP64 from the Zel'dovich particle IC (box 10 Mpc/h, z = 50)
+    through the library's CIC: the adaptive time steps to the reference's
+    end of run (a > 0.5), then the 1000-step split schedule, compared with
+    the oracle's run (tests/golden/long_128_oracle.npz) on its sample cells
+    at steps 100 and 1000."""
+    import os
+    from ifdm_code_amd import icgen
+    F, _O = _mods()
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "long_128_oracle.npz"))
+    n = 128
+    idx = fx["idx"]
+    sched = [tuple(x) for x in fx["sched"]]
+    nad = int(fx["adaptive_steps"])
+    m = -(-len(sched) // nad)
+    _say(f"[long run] n={n} adaptive={nad} split={m} steps={len(sched)}")
+    pos, vel, pm, a0 = icgen.zeldovich(n, 10.0, z=50.0, seed=1234)
+    cfg = _cfg(F, n)
+    with F.Solver(cfg) as s:
+        s.ic_from_particles([(pos, vel)], 10.0e3, pm * 1e10, a0)
+        psi0 = s.get_psi()
+        assert rel_l2(psi0.ravel()[idx], fx["psi0"]) < 1e-12
+        a, ad = s.simulate_native(1.0 / 51, nad)
+        assert a == pytest.approx(float(fx["a_end"]), rel=1e-12)
+        np.testing.assert_allclose(np.array(ad) / m, np.array(sched[::m]), rtol=1e-10)
+        s.set_psi(psi0)
+        s.poisson()
+        done = 0
+        for c in (100, 1000):
+            s.run(sched[done:c])
+            done = c
+            out = s.get_psi().ravel()
+            st = s.poisson()
+            err = rel_l2(out[idx], fx[f"psi{c}"])
+            _say(f"   step {c}: sampled psi rel L2 = {err:.3e}")
+            assert err < 1e-10
+            assert rel_l2(np.abs(out[idx]) ** 2, np.abs(fx[f"psi{c}"]) ** 2) < 1e-10
+            assert abs(st["norm"] / float(fx["norm0"]) - 1) < 1e-12
+            assert st["max_phi"] == pytest.approx(float(fx[f"maxphi{c}"]), rel=1e-10)
+            assert st["max_rho"] == pytest.approx(float(fx[f"maxrho{c}"]), rel=1e-10)
