@@ -48,7 +48,7 @@ def make_ic(F, n: int, xmax_in: float, seed: int = 1234) -> np.ndarray:
     (the reference's IC path).  Returns psi (FP64, host)."""
     from ifdm_code_amd import icgen
     box = 2 * xmax_in * 1e3                                  # kpc/h
-    pos, vel, pm, a = icgen.zeldovich(n, 2 * xmax_in, z=50.0, seed=seed)
+    pos, vel, pm, a = icgen.zeldovich(n, 2 * xmax_in, z=50.0, seed=seed, progress=lambda m: log(f"[bench] {m}"))
     cfg = F.Config.from_cli(2.5e-24, 0.0, xmax_in, n)
     with F.Solver(cfg) as s:
         s.ic_from_particles([(pos, vel)], box, pm * 1e10, a)
@@ -140,6 +140,23 @@ def round_c64(psi: np.ndarray) -> np.ndarray:
     return (psi * f).astype(np.complex64).astype(np.complex128) / f
 
 
+def split_schedule(F, cfg, psi, total: int, a_stop: float = 0.5):
+    """The FP64 adaptive steps (Timestep_criteria each step) up to the
+    reference's own end of run, the first step that takes a past a_stop
+    (src/CosmoBEC_MPI_stan.cpp:250-255); when `total` asks for more steps
+    than that, each is split into m equal parts (D_KE, D_PE, D_QI divided by
+    m) so the same expansion span is covered.  Returns (the first `total`
+    steps, adaptive step count, a at the end of the adaptive run, m)."""
+    sched, a = [], 1.0 / 51
+    with F.Solver(cfg) as sg:
+        sg.set_psi(psi)
+        while a <= a_stop and len(sched) < total:
+            a, d = sg.simulate_native(a, 1)
+            sched += d
+    m = max(1, -(-total // len(sched)))
+    return [tuple(x / m for x in d) for d in sched for _ in range(m)][:total], len(sched), a, m
+
+
 def error_study(F, cfg, cfgm, psi, checkpoints, slab: int, a_stop: float = 0.5) -> dict:
     """Precision error against the FP64 GPU run of the same IC.
 
@@ -153,16 +170,9 @@ def error_study(F, cfg, cfgm, psi, checkpoints, slab: int, a_stop: float = 0.5) 
     the IC alone grows into: the floor the dynamics set) replay that
     schedule; at each checkpoint (cumulative step counts) the per-field
     errors of the latter two against the FP64 path are recorded."""
-    sched, a = [], 1.0 / 51
-    with F.Solver(cfg) as sg:
-        sg.set_psi(psi)
-        while a <= a_stop and len(sched) < max(checkpoints):
-            a, d = sg.simulate_native(a, 1)
-            sched += d
-    m = max(1, -(-max(checkpoints) // len(sched)))
-    steps = [tuple(x / m for x in d) for d in sched for _ in range(m)]
+    steps, nad, a, m = split_schedule(F, cfg, psi, max(checkpoints), a_stop)
     cps = sorted({min(c, len(steps)) for c in checkpoints})
-    log(f"[bench] error study: {len(sched)} adaptive steps to a = {a:.4f}, split {m}x -> {len(steps)} steps")
+    log(f"[bench] error study: {nad} adaptive steps to a = {a:.4f}, split {m}x -> {len(steps)} steps")
     s64, smx, sfl = F.Solver(cfg), F.Solver(cfgm), F.Solver(cfg)
     try:
         for s, p in ((s64, psi), (smx, psi), (sfl, round_c64(psi))):
@@ -183,10 +193,85 @@ def error_study(F, cfg, cfgm, psi, checkpoints, slab: int, a_stop: float = 0.5) 
             log(f"[bench] error after {c} steps: {json.dumps(row)}")
             rows.append(row)
             del psi64, phi64
-        return {"adaptive_steps": len(sched), "a_end": a, "split": m, "steps": len(steps), "checkpoints": rows}
+        return {"adaptive_steps": nad, "a_end": a, "split": m, "steps": len(steps), "checkpoints": rows}
     finally:
         for s in (s64, smx, sfl):
             s.close()
+
+
+def _hip_free_bytes() -> int:
+    """Free device memory (hipMemGetInfo) of the current device."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return free.value
+
+
+def adjoint_study(F, n: int, xmax_in: float, g_in: float, steps: int, K: int, fd_dirs: int) -> dict:
+    """Gradient of L = sum((rho_T - rho_tgt) / rho_bar)^2 with respect to
+    psi_0 (rho_tgt: the final density from the Zel'dovich IC of seed 4321
+    on the same schedule) by the FP64 and the mixed adjoint with device
+    checkpoints every K steps: time, device memory held by the context
+    after the sweep (every buffer it owns), the mixed gradient's error
+    against the FP64 one, and the FP64 gradient against central finite
+    differences (eps = 1e-6) along fd_dirs seeded directions."""
+    psi0 = make_ic(F, n, xmax_in)
+    cfg = F.Config.from_cli(2.5e-24, g_in, xmax_in, n)
+    sched, nad, a_end, m = split_schedule(F, cfg, psi0, steps)
+    log(f"[bench] adjoint study: {len(sched)} steps ({nad} adaptive to a = {a_end:.4f}, split {m}x), K = {K}")
+    with F.Solver(cfg) as s:
+        s.set_psi(make_ic(F, n, xmax_in, seed=4321))
+        s.poisson()
+        s.run(sched)
+        rho_t = np.abs(s.get_psi()) ** 2
+    rbar = float(np.mean(np.abs(psi0) ** 2))
+
+    def loss(p):
+        return float(np.sum(((np.abs(p) ** 2 - rho_t) / rbar) ** 2))
+
+    out, grads = {"steps": len(sched), "K": K, "adaptive_steps": nad, "split": m}, {}
+    for name, prec in (("fp64", F.FP64), ("mixed", F.MIXED)):
+        free0 = _hip_free_bytes()
+        c = F.Config.from_cli(2.5e-24, g_in, xmax_in, n, precision=prec)
+        with F.Solver(c) as s:
+            s.set_psi(psi0)
+            s.poisson()
+            t0 = time.perf_counter()
+            s.run_checkpointed(sched, K)
+            tf = time.perf_counter() - t0
+            psiT = s.get_psi()
+            seed = 4 * (np.abs(psiT) ** 2 - rho_t) / rbar ** 2 * psiT
+            L = loss(psiT)
+            del psiT
+            t0 = time.perf_counter()
+            grads[name] = s.adjoint(sched, seed)
+            ta = time.perf_counter() - t0
+            held = free0 - _hip_free_bytes()
+            del seed
+            r = {"forward_s": round(tf, 3), "adjoint_s": round(ta, 3), "device_bytes_held": held, "loss": L}
+            if name == "fp64" and fd_dirs:
+                rng = np.random.default_rng(77)
+                rels = []
+                for _ in range(fd_dirs):
+                    d = (rng.standard_normal(psi0.shape) + 1j * rng.standard_normal(psi0.shape)) * np.sqrt(rbar / 2)
+                    lp = []
+                    for sgn in (1, -1):
+                        s.set_psi(psi0 + sgn * 1e-6 * d)
+                        s.poisson()
+                        s.run(sched)
+                        lp.append(loss(s.get_psi()))
+                    fd = (lp[0] - lp[1]) / 2e-6
+                    ad = float(np.sum((np.conj(grads[name]) * d).real))
+                    rels.append(abs(ad - fd) / abs(fd))
+                    log(f"[bench] adjoint vs FD: {ad:.12e} {fd:.12e}")
+                    del d
+                r["fd_rel_err_max"] = max(rels)
+            log(f"[bench] adjoint {name}: {json.dumps(r)}")
+            out[name] = r
+    out["mixed_grad_rel_l2_vs_fp64"] = rel_l2(grads["mixed"], grads["fp64"])
+    return out
 
 
 def counter_traffic(n: int, precision: str) -> dict:
@@ -229,6 +314,12 @@ def main():
     ap.add_argument("--error-steps", type=str, default="100,1000",
                     help="comma list of cumulative step counts at which the mixed path's error against the "
                          "FP64 run is reported ('' to skip)")
+    ap.add_argument("--no-mixed-check", action="store_true",
+                    help="skip the mixed path's replay of the timed FP64 schedule (host memory at 1024^3)")
+    ap.add_argument("--adjoint-steps", type=int, default=0,
+                    help="also run the adjoint study over this many steps (0: skip)")
+    ap.add_argument("--adjoint-k", type=int, default=50, help="checkpoint interval of the adjoint study")
+    ap.add_argument("--adjoint-fd", type=int, default=2, help="finite-difference directions of the adjoint study")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -237,7 +328,7 @@ def main():
 
     import ifdm_code_amd as F
     n = args.n
-    xmax_in = 10.0 if n >= 512 else 5.0          # box 20 Mpc/h at 512^3 (10 Mpc/h below)
+    xmax_in = {512: 10.0, 1024: 20.0}.get(n, 5.0)   # box 20 / 40 Mpc/h at 512^3 / 1024^3, else 10 Mpc/h
     cfg0 = F.Config.from_cli(2.5e-24, 0.0, xmax_in, n)
     log(f"[bench] building synthetic IC n={n}")
     t0 = time.perf_counter()
@@ -267,7 +358,8 @@ def main():
         sweep[sv] = round(r[0], 3)
         log(f"[bench] slab {sv}: {r[0]:.2f} steps/s")
     sps, el, avg, sched, psi64, phi64, a = timed_adaptive(F, cfg, psi, args.warmup, args.steps, slab,
-                                                          args.prof_steps, not args.no_mixed)
+                                                          args.prof_steps,
+                                                          not (args.no_mixed or args.no_mixed_check))
     log(f"[bench] FP64: {args.steps} steps in {el:.3f} s ({sps:.2f} steps/s), a = {a:.5f}")
     pb = pass_bytes(n)
     b_alg = sum(pb)
@@ -296,6 +388,15 @@ def main():
         spsm, elm, avgm, _schm, _p, _f, am = timed_adaptive(F, cfgm, psi, args.warmup, args.steps, slab,
                                                             args.prof_steps, False)
         log(f"[bench] mixed: {args.steps} steps in {elm:.3f} s ({spsm:.2f} steps/s)")
+        pbm = pass_bytes(n, 8, 4)
+        out["mixed"] = {
+            "value": round(spsm, 3), "unit": "KDK steps/s", "ms_per_step": round(1e3 * elm / args.steps, 4),
+            "speedup_vs_fp64": round(spsm / sps, 3), "dtype": "c64 storage/FP32 FFT, FP64 phases+sums",
+            "roofline": roofline(n, avgm, 8, 4, "mixed"),
+            "step_hbm_frac_of_8TBs": round(sum(pbm) * spsm / 1e9 / HBM_PEAK_GBS, 4),
+            "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avgm)},
+        }
+    if not args.no_mixed and not args.no_mixed_check:
         # error: the mixed path replays the FP64 run's schedule from the same IC
         with F.Solver(cfgm) as sr:
             sr.set_option("slab_planes", slab)
@@ -308,22 +409,15 @@ def main():
         rho64, rhom = np.abs(psi64) ** 2, np.abs(psim) ** 2
         w = rho64 / rho64.sum()
         ph, _g = _phase_errors(psim, psi64, w)
-        pbm = pass_bytes(n, 8, 4)
-        out["mixed"] = {
-            "value": round(spsm, 3), "unit": "KDK steps/s", "ms_per_step": round(1e3 * elm / args.steps, 4),
-            "speedup_vs_fp64": round(spsm / sps, 3), "dtype": "c64 storage/FP32 FFT, FP64 phases+sums",
-            "roofline": roofline(n, avgm, 8, 4, "mixed"),
-            "step_hbm_frac_of_8TBs": round(sum(pbm) * spsm / 1e9 / HBM_PEAK_GBS, 4),
-            "error_vs_fp64_after_steps": {
-                "steps": len(sched), "density_rel_l2": rel_l2(rhom, rho64),
-                "phase_rho_weighted_rms_rad": ph,
-                "potential_rel_l2": rel_l2(phim, phi64)},
-            "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avgm)},
-        }
+        out["mixed"]["error_vs_fp64_after_steps"] = {
+            "steps": len(sched), "density_rel_l2": rel_l2(rhom, rho64),
+            "phase_rho_weighted_rms_rad": ph, "potential_rel_l2": rel_l2(phim, phi64)}
         del psi64, psim, rho64, rhom, w, phim, phi64
-        cps = [int(x) for x in args.error_steps.split(",") if x.strip()]
-        if cps:
-            out["mixed"]["error_vs_fp64"] = error_study(F, cfg, cfgm, psi, sorted(cps), slab)
+    cps = [int(x) for x in args.error_steps.split(",") if x.strip()]
+    if not args.no_mixed and cps:
+        out["mixed"]["error_vs_fp64"] = error_study(F, cfg, cfgm, psi, sorted(cps), slab)
+    if args.adjoint_steps:
+        out["adjoint"] = adjoint_study(F, n, xmax_in, g_in, args.adjoint_steps, args.adjoint_k, args.adjoint_fd)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(n, psi, g_in, xmax_in, args.cpu_steps)

@@ -35,11 +35,13 @@ def growth_rate(a: float) -> float:
 
 
 def zeldovich(n: int, box_mpc_h: float, z: float = 50.0, seed: int = 1234, slope: float = -2.5,
-              rms_cells: float = 0.2, dtype=None):
+              rms_cells: float = 0.2, dtype=None, progress=None):
     """Returns (pos, vel, pmass, a): pos and vel float32 arrays (n^3, 3) in
     kpc/h and Gadget km/s, the particle mass in 1e10 Msun/h, and a.  The
     field is built in float64 up to 256^3 and in float32 above (memory:
-    about 10 x 4 n^3 bytes at 1024^3)."""
+    about 10 x 4 n^3 bytes at 1024^3).  progress(str), if given, is called
+    after each transform (minutes each at 1024^3)."""
+    say = progress or (lambda _m: None)
     if dtype is None:
         dtype = np.float64 if n <= 256 else np.float32
     a = 1.0 / (1.0 + z)
@@ -55,6 +57,7 @@ def zeldovich(n: int, box_mpc_h: float, z: float = 50.0, seed: int = 1234, slope
     del nz
     dk = np.fft.rfftn(rng.standard_normal((n, n, n), dtype=dtype)) * amp
     del amp
+    say("zeldovich: density field transformed")
     kk[0, 0, 0] = 1.0
     dk /= kk                                    # delta_k / k^2
     del kk
@@ -63,6 +66,7 @@ def zeldovich(n: int, box_mpc_h: float, z: float = 50.0, seed: int = 1234, slope
     for ax in range(3):
         kc = (k1[:, None, None], k1[None, :, None], kz[None, None, :])[ax]
         s[:, ax] = np.fft.irfftn(1j * kc * dk, s=(n, n, n), axes=(0, 1, 2)).ravel()
+        say(f"zeldovich: displacement {ax} done")
     del dk
     rms = math.sqrt(float(np.mean(np.sum(s.astype(np.float64) ** 2, axis=1))))
     s *= np.float32(rms_cells * lg / rms)       # kpc/h
