@@ -16,7 +16,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libifdm.so")
+# IFDM_LIBRARY overrides the library path (e.g. to compare two builds)
+LIB_PATH = os.environ.get("IFDM_LIBRARY") or os.path.join(_HERE, "libifdm.so")
 _LIB = None
 
 FP64 = 0
