@@ -319,7 +319,8 @@ def test_long_run_against_oracle_fixture():
         assert rel_l2(psi0.ravel()[idx], fx["psi0"]) < 1e-12
         a, ad = s.simulate_native(1.0 / 51, nad)
         assert a == pytest.approx(float(fx["a_end"]), rel=1e-12)
-        np.testing.assert_allclose(np.array(ad) / m, np.array(sched[::m]), rtol=1e-10)
+        head = np.array(sched[::m])                  # the fixture keeps the first 1000 split steps
+        np.testing.assert_allclose(np.array(ad)[:len(head)] / m, head, rtol=1e-10)
         s.set_psi(psi0)
         s.poisson()
         done = 0
