@@ -37,6 +37,7 @@ struct Ops {
         z.dtab = nullptr;
         z.k2 = c->k2;
         z.ke_dke = 0.0;
+        z.rev = 0;
         return z;
     }
     static SArgs sargs(ifdm_ctx *c, double dKE) {
@@ -48,6 +49,7 @@ struct Ops {
         s.dKE = dKE;
         s.green = c->green;
         s.fftnorm = c->fftnorm;
+        s.rev = 0;
         return s;
     }
 
@@ -67,13 +69,14 @@ struct Ops {
         return prof_end(c);
     }
     // pass I over x-planes [x0, x0 + nx): the half-length c2r for N >= 32
-    static int c2r(ifdm_ctx *c, int x0 = 0, int nx = 0) {
+    static int c2r(ifdm_ctx *c, int x0 = 0, int nx = 0, int rev = 0) {
+        ZArgs za = zargs(c, 0, 0);
+        za.rev = rev;
         if constexpr (N < 32) {
-            return zpass<Z_C2R>(c, zargs(c, 0, 0), c->part_phi, x0, nx);
+            return zpass<Z_C2R>(c, za, c->part_phi, x0, nx);
         } else {
             constexpr int LZ = c2r_lines<N>();
             if (nx == 0) nx = N;
-            ZArgs za = zargs(c, 0, 0);
             const long l0 = (long)x0 * N, nl = (long)nx * N;
             za.blk0 = (int)(l0 / LZ);
             const int nb = (int)((nl + LZ - 1) / LZ);
@@ -118,8 +121,12 @@ struct Ops {
 
     // One pass of the step by index: 0..8 = A..I, 9 = final reduction.
     // (aphi, arho) is the kick of pass A (both 0: no kick); dKE the drift of C.
+    // B, E and I walk their planes in descending order (the pass before
+    // each of them ascends).
     static int pass(ifdm_ctx *c, int idx, double aphi, double arho, double dKE) {
         const SArgs sa = sargs(c, dKE);
+        SArgs sr = sa;
+        sr.rev = 1;
         c->cur_pass = idx;
         switch (idx) {
             case 0: {
@@ -129,14 +136,18 @@ struct Ops {
                 if (aphi != 0.0 || arho != 0.0) return zpass<Z_KICK_FWD>(c, za, nullptr);
                 return zpass<Z_FWD>(c, za, nullptr);
             }
-            case 1: return spass<1, S_FWD>(c, c->psi, N, N, sa);
+            case 1: return spass<1, S_FWD>(c, c->psi, N, N, sr);
             case 2: return spass<0, S_DRIFT>(c, c->psi, N, N, sa);
             case 3: return spass<1, S_INV>(c, c->psi, N, N, sa);
-            case 4: return zpass<Z_INV_SRC>(c, zargs(c, 0, 0), c->part_rho);
+            case 4: {
+                ZArgs za = zargs(c, 0, 0);
+                za.rev = 1;
+                return zpass<Z_INV_SRC>(c, za, c->part_rho);
+            }
             case 5: return spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sa);
             case 6: return spass<0, S_GREEN>(c, c->shat, c->hs, N / 2 + 1, sa);
             case 7: return spass<1, S_INV>(c, c->shat, c->hs, N / 2 + 1, sa);
-            case 8: return c2r(c, 0, 0);
+            case 8: return c2r(c, 0, 0, 1);
             case 9: {
                 int rc;
                 if ((rc = prof_begin(c))) return rc;

@@ -43,6 +43,7 @@ struct ZArgs {
     void *dtab;       // drift table to fill (block 0 of a forward z pass), or null
     const double *k2; // k_i^2 for the drift table
     double ke_dke;    // ke_coef * dKE for the drift table
+    int rev;          // blocks walk the lines in descending order
 };
 
 struct SArgs {
@@ -53,7 +54,15 @@ struct SArgs {
     double dKE;        // time integral of the drift
     double green;      // -4 pi G
     double fftnorm;    // 1 / N^3
+    int rev;           // blocks walk the fixed axis in descending order
 };
+
+// Block order.  A pass that walks its lines opposite to the pass before it
+// starts on the planes that pass wrote last, which may still sit in the
+// memory-side cache.
+__device__ __forceinline__ int block_order(int rev, unsigned b, unsigned nb) {
+    return rev ? (int)(nb - 1 - b) : (int)b;
+}
 
 enum ZMode { Z_FWD = 0, Z_INV = 1, Z_KICK_FWD = 2, Z_INV_SRC = 3, Z_SRC = 4, Z_C2R = 5, Z_RSRC = 6 };
 enum SMode { S_FWD = 0, S_INV = 1, S_DRIFT = 2, S_GREEN = 3 };
@@ -106,7 +115,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
     constexpr int HS = hs_of(N), NT = LZ * P;
     __shared__ cx<T> lds[LZ * LS];
     const int tid = threadIdx.x, l = tid / P, t = tid - l * P;
-    const int blk = za.blk0 + (int)blockIdx.x;
+    const int blk = za.blk0 + block_order(za.rev, blockIdx.x, gridDim.x);
     const long line = (long)blk * LZ + l;
     const bool valid = line < (long)N * N;
     cx<T> *L = lds + l * LS;
@@ -232,7 +241,7 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
     static_assert(NT == z_lines<N>() * (N / Sched<N>::E), "block shape");
     __shared__ cx<T> lds[(LZ * LS > NT / 2 + 1) ? LZ * LS : NT / 2 + 1];
     const int tid = threadIdx.x, l = tid / P, t = tid - l * P;
-    const int blk = za.blk0 + (int)blockIdx.x;
+    const int blk = za.blk0 + block_order(za.rev, blockIdx.x, gridDim.x);
     const long line = (long)blk * LZ + l;
     const bool valid = line < (long)N * N;
     cx<T> *L = lds + l * LS;
@@ -352,7 +361,7 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     const int k0 = blockIdx.x * TC, k = k0 + c;
-    const int other = sa.other0 + (int)blockIdx.y;
+    const int other = sa.other0 + block_order(sa.rev, blockIdx.y, gridDim.y);
     const bool valid = k < kv;
     long base0, stride;
     spass_addr<N, AXIS>(other, k0, rl, base0, stride);
