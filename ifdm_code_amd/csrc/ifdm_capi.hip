@@ -90,15 +90,17 @@ struct Ops {
     }
 
     // strided pass; for AXIS 1 the fixed index (x) runs over [o0, o0 + no)
-    template <int AXIS, int MODE>
+    // WIDE: 256-byte row tiles in FP64 (the psi passes; the half spectrum
+    // keeps 128-byte tiles, whose last tile holds a single valid column)
+    template <int AXIS, int MODE, bool WIDE = false>
     static int spass(ifdm_ctx *c, void *arr, int rl, int kv, SArgs sa, int o0 = 0, int no = 0) {
-        constexpr int TC = s_cols<T>();
         if (no == 0) no = N;
         sa.other0 = o0;
-        const int ntz = (kv + TC - 1) / TC;
         int rc;
         if ((rc = prof_begin(c))) return rc;
         if (c->persistent & (AXIS == 1 ? 1 : 2)) {
+            constexpr int TC = s_cols<T, N>();
+            const int ntz = (kv + TC - 1) / TC;
             static int occ = -1;   // resident work-groups per CU for this instantiation
             if (occ < 0) {
                 int o = 0;
@@ -112,7 +114,9 @@ struct Ops {
             hipLaunchKernelGGL((k_spass_pp<T, N, AXIS, MODE>), dim3(nb), dim3(TC * P), 0, c->stream,
                                (cx<T> *)arr, rl, kv, (const cx<T> *)c->tw, sa, ntz, ntiles);
         } else {
-            hipLaunchKernelGGL((k_spass<T, N, AXIS, MODE>), dim3(ntz, no), dim3(TC * P), 0, c->stream,
+            constexpr int TC = s_cols<T, N, WIDE>();
+            const int ntz = (kv + TC - 1) / TC;
+            hipLaunchKernelGGL((k_spass<T, N, AXIS, MODE, WIDE>), dim3(ntz, no), dim3(TC * P), 0, c->stream,
                                (cx<T> *)arr, rl, kv, (const cx<T> *)c->tw, sa);
         }
         LAUNCHCHK(c);
@@ -136,9 +140,9 @@ struct Ops {
                 if (aphi != 0.0 || arho != 0.0) return zpass<Z_KICK_FWD>(c, za, nullptr);
                 return zpass<Z_FWD>(c, za, nullptr);
             }
-            case 1: return spass<1, S_FWD>(c, c->psi, N, N, sr);
-            case 2: return spass<0, S_DRIFT>(c, c->psi, N, N, sa);
-            case 3: return spass<1, S_INV>(c, c->psi, N, N, sa);
+            case 1: return spass<1, S_FWD, true>(c, c->psi, N, N, sr);
+            case 2: return spass<0, S_DRIFT, true>(c, c->psi, N, N, sa);
+            case 3: return spass<1, S_INV, true>(c, c->psi, N, N, sa);
             case 4: {
                 ZArgs za = zargs(c, 0, 0);
                 za.rev = 1;
@@ -231,14 +235,14 @@ struct Ops {
             c->cur_pass = 0;
             rc = kick ? zpass<Z_KICK_FWD>(c, za, nullptr, x0, nx) : zpass<Z_FWD>(c, za, nullptr, x0, nx);
             c->cur_pass = 1;
-            if (rc || (rc = spass<1, S_FWD>(c, c->psi, N, N, sa, x0, nx))) return rc;
+            if (rc || (rc = spass<1, S_FWD, true>(c, c->psi, N, N, sa, x0, nx))) return rc;
         }
         c->cur_pass = 2;
-        if ((rc = spass<0, S_DRIFT>(c, c->psi, N, N, sa))) return rc;
+        if ((rc = spass<0, S_DRIFT, true>(c, c->psi, N, N, sa))) return rc;
         for (int x0 = 0; x0 < N; x0 += S) {
             const int nx = std::min(S, N - x0);
             c->cur_pass = 3;
-            if ((rc = spass<1, S_INV>(c, c->psi, N, N, sa, x0, nx))) return rc;
+            if ((rc = spass<1, S_INV, true>(c, c->psi, N, N, sa, x0, nx))) return rc;
             c->cur_pass = 4;
             if ((rc = zpass<Z_INV_SRC>(c, z0, c->part_rho, x0, nx))) return rc;
             c->cur_pass = 5;
@@ -274,9 +278,9 @@ struct Ops {
         const ZArgs z0 = zargs(c, 0, 0);
         int rc;
         if ((rc = zpass<Z_FWD>(c, z0, nullptr))) return rc;
-        if ((rc = spass<1, S_FWD>(c, c->psi, N, N, sa))) return rc;
-        if ((rc = spass<0, S_GREEN>(c, c->psi, N, N, sa))) return rc;
-        if ((rc = spass<1, S_INV>(c, c->psi, N, N, sa))) return rc;
+        if ((rc = spass<1, S_FWD, true>(c, c->psi, N, N, sa))) return rc;
+        if ((rc = spass<0, S_GREEN, true>(c, c->psi, N, N, sa))) return rc;
+        if ((rc = spass<1, S_INV, true>(c, c->psi, N, N, sa))) return rc;
         return zpass<Z_INV>(c, z0, nullptr);
     }
 
@@ -284,9 +288,9 @@ struct Ops {
         SArgs sa = sargs(c, dKE);
         int rc;
         if ((rc = pass(c, 0, 0.0, 0.0, dKE))) return rc;
-        if ((rc = spass<1, S_FWD>(c, c->psi, N, N, sa))) return rc;
-        if ((rc = spass<0, S_DRIFT>(c, c->psi, N, N, sa))) return rc;
-        if ((rc = spass<1, S_INV>(c, c->psi, N, N, sa))) return rc;
+        if ((rc = spass<1, S_FWD, true>(c, c->psi, N, N, sa))) return rc;
+        if ((rc = spass<0, S_DRIFT, true>(c, c->psi, N, N, sa))) return rc;
+        if ((rc = spass<1, S_INV, true>(c, c->psi, N, N, sa))) return rc;
         return zpass<Z_INV>(c, zargs(c, 0, 0), nullptr);
     }
 
@@ -295,12 +299,12 @@ struct Ops {
         int rc;
         if (sign < 0) {
             if ((rc = zpass<Z_FWD>(c, zargs(c, 0, 0), nullptr))) return rc;
-            if ((rc = spass<1, S_FWD>(c, c->psi, N, N, sa))) return rc;
-            return spass<0, S_FWD>(c, c->psi, N, N, sa);
+            if ((rc = spass<1, S_FWD, true>(c, c->psi, N, N, sa))) return rc;
+            return spass<0, S_FWD, true>(c, c->psi, N, N, sa);
         }
         if ((rc = zpass<Z_INV>(c, zargs(c, 0, 0), nullptr))) return rc;
-        if ((rc = spass<1, S_INV>(c, c->psi, N, N, sa))) return rc;
-        return spass<0, S_INV>(c, c->psi, N, N, sa);
+        if ((rc = spass<1, S_INV, true>(c, c->psi, N, N, sa))) return rc;
+        return spass<0, S_INV, true>(c, c->psi, N, N, sa);
     }
 
     static int kick(ifdm_ctx *c, double aphi, double arho) {

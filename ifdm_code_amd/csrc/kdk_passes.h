@@ -71,7 +71,15 @@ template <int N> constexpr int z_lines() {
     constexpr int P = N / Sched<N>::E;
     return (256 / P) < (N * N) ? (256 / P) : (N * N);
 }
-template <typename T> constexpr int s_cols() { return 128 / (int)sizeof(cx<T>); }  // 128-byte rows
+// z-columns per strided-pass tile: 128-byte row segments (8 complex
+// doubles, 16 complex floats); WIDE doubles them for FP64 (256-byte rows,
+// used on psi), and a work-group never exceeds 1024 threads
+template <typename T, int N, bool WIDE = false> constexpr int s_cols() {
+    constexpr int base = 128 / (int)sizeof(cx<T>);
+    constexpr int want = (WIDE && sizeof(cx<T>) == 16) ? 2 * base : base;
+    constexpr int cap = 1024 / (N / Sched<N>::E);
+    return want < cap ? want : cap;
+}
 
 // exp(-i th) in the storage precision T: the FP64 argument is reduced mod
 // pi/2 in FP64, then sin/cos are evaluated in T (fastmath.h)
@@ -354,10 +362,10 @@ template <typename T, int N> struct TileIO {
 };
 
 // one tile per work-group: grid (z-column tiles, lines of the fixed axis)
-template <typename T, int N, int AXIS, int MODE>
-__global__ __launch_bounds__(s_cols<T>() * (N / Sched<N>::E))
+template <typename T, int N, int AXIS, int MODE, bool WIDE>
+__global__ __launch_bounds__((s_cols<T, N, WIDE>() * (N / Sched<N>::E)))
 void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw, SArgs sa) {
-    constexpr int E = Sched<N>::E, P = N / E, TC = s_cols<T>(), LS = lds_line<N>();
+    constexpr int E = Sched<N>::E, P = N / E, TC = s_cols<T, N, WIDE>(), LS = lds_line<N>();
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     const int k0 = blockIdx.x * TC, k = k0 + c;
@@ -385,10 +393,10 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
 // overlap the butterflies and LDS exchanges.  The first LDS write of every
 // transform is preceded by a barrier, so consecutive tiles never race on LDS.
 template <typename T, int N, int AXIS, int MODE>
-__global__ __launch_bounds__(s_cols<T>() * (N / Sched<N>::E))
+__global__ __launch_bounds__((s_cols<T, N>() * (N / Sched<N>::E)))
 void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw, SArgs sa, int ntz,
                 int ntiles) {
-    constexpr int E = Sched<N>::E, P = N / E, TC = s_cols<T>(), LS = lds_line<N>();
+    constexpr int E = Sched<N>::E, P = N / E, TC = s_cols<T, N>(), LS = lds_line<N>();
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     cx<T> *L = lds + c * LS;

@@ -336,3 +336,34 @@ def test_long_run_against_oracle_fixture():
             assert abs(st["norm"] / float(fx["norm0"]) - 1) < 1e-12
             assert st["max_phi"] == pytest.approx(float(fx[f"maxphi{c}"]), rel=1e-10)
             assert st["max_rho"] == pytest.approx(float(fx[f"maxrho{c}"]), rel=1e-10)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_plane_wave_drift_1024(precision):
+    """The largest supported mesh, 1024^3, in both precisions: a plane wave
+    psi = A exp(i k.x) only changes phase under the drift, by
+    exp(-i hbar |k|^2 dKE / (2 m)) (src/struct3D.h:121), uniformly."""
+    import math
+    F, O = _mods()
+    n = 1024
+    _say(f"[plane wave] n={n} precision={precision}")
+    rp = _rp(O, n)
+    cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n, precision=precision)
+    kv = O.kvec(n, rp.xmax)
+    m = (3, -5, 7)
+    x = rp.xmax / n - rp.xmax + np.arange(n) * rp.dx
+    amp = math.sqrt(rp.rhoc / rp.mfdm_solar)
+    ex, ey, ez = (np.exp(1j * kv[mi % n] * (x + rp.xmax)) for mi in m)
+    psi = (amp * ex)[:, None, None] * ey[None, :, None] * ez[None, None, :]
+    k2 = sum(kv[mi % n] ** 2 for mi in m)
+    dKE = 0.3 / (0.5 * O.hbar / (rp.mfdm / O.c_light ** 2) * k2)         # 0.3 rad
+    with F.Solver(cfg) as s:
+        s.set_psi(psi)
+        s.drift(dKE)
+        out = s.get_psi()
+    want = complex(math.cos(0.3), -math.sin(0.3))
+    err = 0.0
+    for i in range(0, n, 128):                    # chunks keep the temporaries small
+        err = max(err, float(np.max(np.abs(out[i:i + 128] / psi[i:i + 128] - want))))
+    _say(f"   max |psi'/psi - exp(-i 0.3)| = {err:.3e}")
+    assert err < (1e-11 if precision == 0 else 2e-5)
