@@ -304,10 +304,12 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
 // wavefront covers 128-byte row segments.
 
 // the transform (and the spectral multiply of passes C and G) of one tile;
-// FRESH: the first transform of the work-group (no earlier LDS use)
+// FRESH: the first transform of the work-group (no earlier LDS use).  tab
+// and k2 are the drift table and k_i^2 (global memory, or LDS copies).
 template <typename T, int N, int MODE, bool FRESH>
 __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &tw,
-                                              const SArgs &sa, int other, int k, bool valid) {
+                                              const SArgs &sa, int other, int k, bool valid,
+                                              const cx<T> *tab, const double *k2) {
     constexpr int E = Sched<N>::E, P = N / E;
     if constexpr (MODE == S_FWD) fft_line<T, N, -1, false, FRESH>(v, L, t, tw);
     else if constexpr (MODE == S_INV) fft_line<T, N, +1, false, FRESH>(v, L, t, tw);
@@ -315,16 +317,15 @@ __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const T
         fft_line<T, N, -1, false, FRESH>(v, L, t, tw);
         // spectral indices of this thread's points: along the line i, fixed `other` and k
         if constexpr (MODE == S_DRIFT) {
-            const cx<T> *tab = (const cx<T> *)sa.dtab;
             const cx<T> w = cscale(cmul(tab[other], tab[valid ? k : 0]), (T)sa.fftnorm);
 #pragma unroll
             for (int e = 0; e < E; ++e) v[e] = cmul(v[e], cmul(tab[t + e * P], w));
         } else {
-            const double k2o = sa.k2[other], k2k = valid ? sa.k2[k] : 0.0;
+            const double k2o = k2[other], k2k = valid ? k2[k] : 0.0;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                const double k2 = sa.k2[t + e * P] + k2o + k2k;
-                const double gf = (k2 == 0.0) ? 0.0 : sa.green / k2 * sa.fftnorm;
+                const double kk = k2[t + e * P] + k2o + k2k;
+                const double gf = (kk == 0.0) ? 0.0 : sa.green / kk * sa.fftnorm;
                 v[e] = cscale(v[e], (T)gf);
             }
         }
@@ -380,7 +381,7 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = valid ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
-    spass_compute<T, N, MODE, true>(v, L, t, tb, sa, other, k, valid);
+    spass_compute<T, N, MODE, true>(v, L, t, tb, sa, other, k, valid, (const cx<T> *)sa.dtab, sa.k2);
     if (valid) {
 #pragma unroll
         for (int e = 0; e < E; ++e) io.store(c + (long)(t + e * P) * stride, v[e]);
@@ -397,12 +398,34 @@ __global__ __launch_bounds__((s_cols<T, N>() * (N / Sched<N>::E)))
 void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw, SArgs sa, int ntz,
                 int ntiles) {
     constexpr int E = Sched<N>::E, P = N / E, TC = s_cols<T, N>(), LS = lds_line<N>();
+    constexpr int NT = TC * P;
     __shared__ cx<T> lds[TC * LS];
+    // the spectral multiply's table in LDS, so no global load is issued
+    // after a tile's prefetch (loads complete in order: waiting for a table
+    // value would also wait for the prefetch)
+    // (up to N = 512; the 1024-point exchange buffer leaves no room)
+    constexpr bool LT = N <= 512;
+    __shared__ cx<T> tabL[(LT && MODE == S_DRIFT) ? N : 1];
+    __shared__ double k2L[(LT && MODE == S_GREEN) ? N : 1];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     cx<T> *L = lds + c * LS;
     int tile = blockIdx.x;
     if (tile >= ntiles) return;
+    if constexpr (LT && MODE == S_DRIFT) {
+        for (int i = tid; i < N; i += NT) tabL[i] = ((const cx<T> *)sa.dtab)[i];
+    }
+    if constexpr (LT && MODE == S_GREEN) {
+        for (int i = tid; i < N; i += NT) k2L[i] = sa.k2[i];
+    }
+    const cx<T> *tab = LT ? (const cx<T> *)tabL : (const cx<T> *)sa.dtab;
+    const double *k2t = LT ? (const double *)k2L : sa.k2;
     const long stride = AXIS == 1 ? (long)rl : (long)N * rl;
+    // With at least TC columns per line every column of every tile lies
+    // inside its row (rl >= the tile span; columns k >= kv are the half
+    // spectrum's zero padding), so loads and stores need no condition and
+    // the count of memory operations in flight is the same on every path:
+    // the wait for the prefetch then lets this tile's stores stay in flight.
+    constexpr bool FULL = N >= TC;
     TwBase<T, N> tb;
     load_twiddles<T, N>(tb, t, tw);
     cx<T> v[E], nx[E];
@@ -413,32 +436,41 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__
         const TileIO<T, N> io(a, b0);
         const bool valid = k0 + c < kv;
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = valid ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
+        for (int e = 0; e < E; ++e) v[e] = (FULL || valid) ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
     }
+    // nothing in flight on entry to the loop: the waits the compiler places
+    // in the loop then follow the steady state alone (this tile's stores may
+    // stay in flight while the next tile's loads are awaited)
+    __builtin_amdgcn_s_waitcnt(0);
     for (;;) {
         const int nt = tile + (int)gridDim.x;
         const bool more = nt < ntiles;
-        if (more) {
-            const int o2 = sa.other0 + nt / ntz, k2 = (nt % ntz) * TC;
+        {
+            // the last tile re-reads itself, so every iteration issues the same
+            // number of loads (the compiler's waits then count exactly)
+            const int lt = more ? nt : tile;
+            const int o2 = sa.other0 + lt / ntz, k2 = (lt % ntz) * TC;
             long b2, st;
             spass_addr<N, AXIS>(o2, k2, rl, b2, st);
             const TileIO<T, N> io(a, b2);
             const bool valid2 = k2 + c < kv;
 #pragma unroll
-            for (int e = 0; e < E; ++e) nx[e] = valid2 ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
+            for (int e = 0; e < E; ++e) nx[e] = (FULL || valid2) ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
         }
+        __builtin_amdgcn_sched_barrier(0);   // issue the prefetch before the transform
         const int other = sa.other0 + tile / ntz, k0 = (tile % ntz) * TC, k = k0 + c;
         const bool valid = k < kv;
-        spass_compute<T, N, MODE, false>(v, L, t, tb, sa, other, k, valid);
+        spass_compute<T, N, MODE, false>(v, L, t, tb, sa, other, k, valid, tab, k2t);
         {
             long b0, st;
             spass_addr<N, AXIS>(other, k0, rl, b0, st);
             const TileIO<T, N> io(a, b0);
-            if (valid) {
+            if (FULL || valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) io.store(c + (long)(t + e * P) * stride, v[e]);
             }
         }
+        __builtin_amdgcn_sched_barrier(0);   // take over the prefetch only after the stores
         if (!more) break;
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = nx[e];
