@@ -55,7 +55,7 @@ struct ifdm_ctx {
     // chains (A,B), (D,E,F), (H,I) run slab by slab, S x-planes per slab
     int slab = 0;
     // strided passes with persistent prefetching work-groups: bit 0 y, bit 1 x
-    int persistent = 3;
+    int persistent = 0;
     int num_cu = 0;
     // optional live per-pass timing (ifdm_set_profiling)
     bool prof = false;
