@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "KDK steps/sec at 512^3: FP64 vs FP32-mixed at stated error; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-PASS_NAMES = ["A_kick_zfft", "B_yfft", "C_xfft_drift_xifft", "D_yifft", "E_zifft_src_r2c",
+PASS_NAMES = ["T_drift_table", "A_kick_zdrift", "B_ydrift", "C_xdrift_src", "E_src_r2c_z",
               "F_yfft_half", "G_xfft_green_xifft_half", "H_yifft_half", "I_c2r_z", "reduce"]
 
 
@@ -33,12 +33,21 @@ def log(*a):
 
 
 def pass_bytes(n: int, sc: int = 16, sr: int = 8) -> list[float]:
-    """Algorithmic HBM bytes of each pass: one read and one write of the
-    field it transforms; s_c complex, s_r real element size; h = (N/2+1)/N."""
+    """Algorithmic HBM bytes of each pass: one read and one write of each
+    field it touches; s_c complex, s_r real element size; h = (N/2+1)/N.
+    A, B, C each run one axis's drift round trip on psi (the drift factor
+    separates by axis), C also writes the real Poisson source."""
     n3 = float(n) ** 3
     h = (n // 2 + 1) / n
-    return [n3 * (2 * sc + sr), n3 * 2 * sc, n3 * 2 * sc, n3 * 2 * sc, n3 * (2 * sc + h * sc),
+    return [0.0, n3 * (2 * sc + sr), n3 * 2 * sc, n3 * (2 * sc + sr), n3 * (sr + h * sc),
             n3 * 2 * h * sc, n3 * 2 * h * sc, n3 * 2 * h * sc, n3 * (h * sc + sr), 0.0]
+
+
+def survey_model_bytes(n: int, sc: int = 16) -> float:
+    """B_alg of the minimal five-pass-drift pencil schedule, (11 + 8h) s_c N^3:
+    the reference point of the step's HBM roofline fraction."""
+    h = (n // 2 + 1) / n
+    return (11 + 8 * h) * sc * float(n) ** 3
 
 
 def make_ic(F, n: int, xmax_in: float, seed: int = 1234) -> np.ndarray:
@@ -80,14 +89,13 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-def timed_adaptive(F, cfg, psi, warmup: int, steps: int, slab: int, prof_steps: int, keep_state: bool):
+def timed_adaptive(F, cfg, psi, warmup: int, steps: int, prof_steps: int, keep_state: bool):
     """warm-up, then `steps` timed adaptive steps (ifdm_simulate, no events),
     then `prof_steps` more with live per-launch HIP events for the per-pass
     times.  Returns (steps/s, seconds, avg ms per pass, schedule of the
     warm-up + timed steps, psi and phi after them (or None), a)."""
     s = F.Solver(cfg)
     s.set_psi(psi)
-    s.set_option("slab_planes", slab)
     a, sched_w = s.simulate_native(1.0 / 51, warmup)
     t0 = time.perf_counter()
     a, sched_t = s.simulate_native(a, steps)
@@ -157,7 +165,7 @@ def split_schedule(F, cfg, psi, total: int, a_stop: float = 0.5):
     return [tuple(x / m for x in d) for d in sched for _ in range(m)][:total], len(sched), a, m
 
 
-def error_study(F, cfg, cfgm, psi, checkpoints, slab: int, a_stop: float = 0.5) -> dict:
+def error_study(F, cfg, cfgm, psi, checkpoints, a_stop: float = 0.5) -> dict:
     """Precision error against the FP64 GPU run of the same IC.
 
     Schedule: the FP64 adaptive run (Timestep_criteria each step) up to the
@@ -176,7 +184,6 @@ def error_study(F, cfg, cfgm, psi, checkpoints, slab: int, a_stop: float = 0.5) 
     s64, smx, sfl = F.Solver(cfg), F.Solver(cfgm), F.Solver(cfg)
     try:
         for s, p in ((s64, psi), (smx, psi), (sfl, round_c64(psi))):
-            s.set_option("slab_planes", slab)
             s.set_psi(p)
             s.poisson()
         done, rows = 0, []
@@ -291,7 +298,7 @@ def counter_traffic(n: int, precision: str) -> dict:
 
 def roofline(n: int, avg: list, sc: int, sr: int, precision: str) -> dict:
     pb = pass_bytes(n, sc, sr)
-    dom = int(np.argmax(avg[:9]))
+    dom = int(np.argmax(avg[:9]))          # the passes; the last entry is the reduction
     achieved = pb[dom] / (avg[dom] * 1e-3) / 1e9
     traffic = counter_traffic(n, precision).get(PASS_NAMES[dom])
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -308,9 +315,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--slab", type=int, default=-1, help="x-planes per slab chain (-1: library default)")
     ap.add_argument("--prof-steps", type=int, default=5)
-    ap.add_argument("--sweep-slab", type=str, default="", help="comma list of slab sizes to time (FP64)")
     ap.add_argument("--error-steps", type=str, default="100,1000",
                     help="comma list of cumulative step counts at which the mixed path's error against the "
                          "FP64 run is reported ('' to skip)")
@@ -351,13 +356,7 @@ def main():
     cfg = F.Config.from_cli(2.5e-24, g_in, xmax_in, n)
     log(f"[bench] g = {g_in:.4e} eV cm^3")
 
-    slab = args.slab if args.slab >= 0 else F.default_slab(n)
-    sweep = {}
-    for sv in [int(x) for x in args.sweep_slab.split(",") if x.strip()]:
-        r = timed_adaptive(F, cfg, psi, args.warmup, args.steps, sv, 0, False)
-        sweep[sv] = round(r[0], 3)
-        log(f"[bench] slab {sv}: {r[0]:.2f} steps/s")
-    sps, el, avg, sched, psi64, phi64, a = timed_adaptive(F, cfg, psi, args.warmup, args.steps, slab,
+    sps, el, avg, sched, psi64, phi64, a = timed_adaptive(F, cfg, psi, args.warmup, args.steps,
                                                           args.prof_steps,
                                                           not (args.no_mixed or args.no_mixed_check))
     log(f"[bench] FP64: {args.steps} steps in {el:.3f} s ({sps:.2f} steps/s), a = {a:.5f}")
@@ -372,20 +371,23 @@ def main():
                 "z=50, seed 1234) CIC-deposited into psi by the library",
         "config": {"workload": f"KDK step at {n}^3, box {2 * xmax_in:g} Mpc/h, z=50, mfdm=2.5e-24 eV, "
                                f"g={g_in:.3e} eV cm^3 (1% of gravity), adaptive dt, FP64",
-                   "n": n, "precision": "fp64", "slab_planes": slab},
+                   "n": n, "precision": "fp64"},
         "roofline": roofline(n, avg, 16, 8, "fp64"),
-        "step_hbm": {"alg_bytes_per_step": b_alg, "achieved_GBs": round(step_gbs, 1),
-                     "frac_of_8TBs": round(step_gbs / HBM_PEAK_GBS, 4),
+        # frac_of_8TBs: the step on the reference five-pass-drift schedule's
+        # bytes, (11 + 8h) s_c N^3 per step at 8 TB/s; bytes_moved_*: this
+        # schedule's own algorithmic bytes (fewer: three drift passes)
+        "step_hbm": {"alg_bytes_per_step": survey_model_bytes(n), "frac_of_8TBs":
+                         round(survey_model_bytes(n) * sps / 1e9 / HBM_PEAK_GBS, 4),
+                     "bytes_moved_per_step": b_alg, "achieved_GBs": round(step_gbs, 1),
+                     "bytes_moved_frac_of_8TBs": round(step_gbs / HBM_PEAK_GBS, 4),
                      "cell_updates_per_s": round(sps * n ** 3, 1),
                      "stream_copy_GBs": round(copy_gbs, 1)},
         "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avg)},
     }
-    if sweep:
-        out["slab_sweep_steps_per_s"] = sweep
 
     if not args.no_mixed:
         cfgm = F.Config.from_cli(2.5e-24, g_in, xmax_in, n, precision=F.MIXED)
-        spsm, elm, avgm, _schm, _p, _f, am = timed_adaptive(F, cfgm, psi, args.warmup, args.steps, slab,
+        spsm, elm, avgm, _schm, _p, _f, am = timed_adaptive(F, cfgm, psi, args.warmup, args.steps,
                                                             args.prof_steps, False)
         log(f"[bench] mixed: {args.steps} steps in {elm:.3f} s ({spsm:.2f} steps/s)")
         pbm = pass_bytes(n, 8, 4)
@@ -393,13 +395,13 @@ def main():
             "value": round(spsm, 3), "unit": "KDK steps/s", "ms_per_step": round(1e3 * elm / args.steps, 4),
             "speedup_vs_fp64": round(spsm / sps, 3), "dtype": "c64 storage/FP32 FFT, FP64 phases+sums",
             "roofline": roofline(n, avgm, 8, 4, "mixed"),
-            "step_hbm_frac_of_8TBs": round(sum(pbm) * spsm / 1e9 / HBM_PEAK_GBS, 4),
+            "step_hbm_frac_of_8TBs": round(survey_model_bytes(n, 8) * spsm / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes_moved_frac_of_8TBs": round(sum(pbm) * spsm / 1e9 / HBM_PEAK_GBS, 4),
             "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avgm)},
         }
     if not args.no_mixed and not args.no_mixed_check:
         # error: the mixed path replays the FP64 run's schedule from the same IC
         with F.Solver(cfgm) as sr:
-            sr.set_option("slab_planes", slab)
             sr.set_psi(psi)
             sr.poisson()
             sr.run([tuple(x) for x in sched])
@@ -415,7 +417,7 @@ def main():
         del psi64, psim, rho64, rhom, w, phim, phi64
     cps = [int(x) for x in args.error_steps.split(",") if x.strip()]
     if not args.no_mixed and cps:
-        out["mixed"]["error_vs_fp64"] = error_study(F, cfg, cfgm, psi, sorted(cps), slab)
+        out["mixed"]["error_vs_fp64"] = error_study(F, cfg, cfgm, psi, sorted(cps))
     if args.adjoint_steps:
         out["adjoint"] = adjoint_study(F, n, xmax_in, g_in, args.adjoint_steps, args.adjoint_k, args.adjoint_fd)
     if not args.no_cpu_baseline:

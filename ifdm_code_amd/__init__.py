@@ -196,11 +196,6 @@ def timestep_criteria(cfg: Config, a: float, stats: dict):
     return (dt.dKE, dt.dPE, dt.dQI), an.value, which.value
 
 
-def default_slab(n: int) -> int:
-    """x-planes per slab chain used by the bench unless told otherwise."""
-    return 0
-
-
 def scale_integral(a0: float, a1: float, power: int) -> float:
     return lib().ifdm_scale_integral(a0, a1, power)
 

@@ -26,6 +26,7 @@ struct ifdm_ctx {
     void *dtab = nullptr;      // N complex: per-step drift table
     double *red_host = nullptr;  // 3 doubles, pinned
     int nz_blocks = 0;
+    int rho_parts = 0;   // density partials written by the last source pass
     // physics
     double m_over_hbar = 0;    // (mfdm / c^2) / hbar
     double g_over_hbar = 0;
@@ -51,9 +52,6 @@ struct ifdm_ctx {
     void *adj_g = nullptr, *adj_s = nullptr, *adj_phs = nullptr;
     void *ck = nullptr;
     int ck_k = 0, ck_count = 0, ck_nsteps = -1;
-    // launch schedule: 0 = one launch per pass; S > 0 = the x-slab-local
-    // chains (A,B), (D,E,F), (H,I) run slab by slab, S x-planes per slab
-    int slab = 0;
     // strided passes with persistent prefetching work-groups: bit 0 y, bit 1 x
     int persistent = 0;
     int num_cu = 0;

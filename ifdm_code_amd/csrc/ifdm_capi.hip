@@ -35,8 +35,6 @@ struct Ops {
         z.ref_range = c->cfg.ref_maxphi_range;
         z.blk0 = 0;
         z.dtab = nullptr;
-        z.k2 = c->k2;
-        z.ke_dke = 0.0;
         z.rev = 0;
         return z;
     }
@@ -50,6 +48,10 @@ struct Ops {
         s.green = c->green;
         s.fftnorm = c->fftnorm;
         s.rev = 0;
+        s.ms = c->ms * c->psc2;
+        s.rhoc = c->mixed ? c->rho_mean : c->cfg.rhoc;
+        s.src = c->phi;
+        s.part = c->part_rho;
         return s;
     }
 
@@ -98,7 +100,10 @@ struct Ops {
         sa.other0 = o0;
         int rc;
         if ((rc = prof_begin(c))) return rc;
-        if (c->persistent & (AXIS == 1 ? 1 : 2)) {
+        bool done = false;
+        if constexpr (MODE != S_DRIFT1_SRC) {
+          if (c->persistent & (AXIS == 1 ? 1 : 2)) {
+            done = true;
             constexpr int TC = s_cols<T, N>();
             const int ntz = (kv + TC - 1) / TC;
             static int occ = -1;   // resident work-groups per CU for this instantiation
@@ -113,9 +118,12 @@ struct Ops {
             const int nb = std::min(ntiles, occ * c->num_cu);
             hipLaunchKernelGGL((k_spass_pp<T, N, AXIS, MODE>), dim3(nb), dim3(TC * P), 0, c->stream,
                                (cx<T> *)arr, rl, kv, (const cx<T> *)c->tw, sa, ntz, ntiles);
-        } else {
+          }
+        }
+        if (!done) {
             constexpr int TC = s_cols<T, N, WIDE>();
             const int ntz = (kv + TC - 1) / TC;
+            if constexpr (MODE == S_DRIFT1_SRC) c->rho_parts = ntz * no;
             hipLaunchKernelGGL((k_spass<T, N, AXIS, MODE, WIDE>), dim3(ntz, no), dim3(TC * P), 0, c->stream,
                                (cx<T> *)arr, rl, kv, (const cx<T> *)c->tw, sa);
         }
@@ -123,10 +131,20 @@ struct Ops {
         return prof_end(c);
     }
 
-    // One pass of the step by index: 0..8 = A..I, 9 = final reduction.
-    // (aphi, arho) is the kick of pass A (both 0: no kick); dKE the drift of C.
-    // B, E and I walk their planes in descending order (the pass before
-    // each of them ascends).
+    // The drift factor of the step separates by axis (it is exp(-i c k^2
+    // dKE) with k^2 = k_x^2 + k_y^2 + k_z^2), so each axis runs its own
+    // transform - factor - inverse round trip and psi returns to real space
+    // after three passes instead of five.  One pass of the step by index:
+    //   0 T  the per-axis drift table E[i] = exp(-i c k_i^2 dKE)
+    //   1 A  z: kick (aphi, arho; both 0: none), forward, E[k_z]/N, inverse
+    //   2 B  y: forward, E[k_y]/N, inverse
+    //   3 C  x: forward, E[k_x]/N, inverse; Poisson source s = m_s|psi|^2 -
+    //        rho_c into phi's buffer; max and sum of |psi|^2
+    //   4 E  z: real-to-complex transform of s into the half spectrum
+    //   5 F, 6 G, 7 H, 8 I: the Poisson solve on the half spectrum -> phi
+    //   9    final reduction of the partials
+    // B, F and I walk their planes in descending order (the pass before
+    // each of them ascends), starting on what is still in the memory-side cache.
     static int pass(ifdm_ctx *c, int idx, double aphi, double arho, double dKE) {
         const SArgs sa = sargs(c, dKE);
         SArgs sr = sa;
@@ -134,21 +152,27 @@ struct Ops {
         c->cur_pass = idx;
         switch (idx) {
             case 0: {
+                int rc;
+                if ((rc = prof_begin(c))) return rc;
+                hipLaunchKernelGGL((k_drift_table<T, N>), dim3(1), dim3(256), 0, c->stream, (cx<T> *)c->dtab,
+                                   (const double *)c->k2, c->ke_coef * dKE);
+                LAUNCHCHK(c);
+                return prof_end(c);
+            }
+            case 1: {
                 ZArgs za = zargs(c, aphi, arho);
                 za.dtab = c->dtab;
-                za.ke_dke = c->ke_coef * dKE;
-                if (aphi != 0.0 || arho != 0.0) return zpass<Z_KICK_FWD>(c, za, nullptr);
-                return zpass<Z_FWD>(c, za, nullptr);
+                if (aphi != 0.0 || arho != 0.0) return zpass<Z_KICK_DRIFT>(c, za, nullptr);
+                return zpass<Z_DRIFT>(c, za, nullptr);
             }
-            case 1: return spass<1, S_FWD, true>(c, c->psi, N, N, sr);
-            case 2: return spass<0, S_DRIFT, true>(c, c->psi, N, N, sa);
-            case 3: return spass<1, S_INV, true>(c, c->psi, N, N, sa);
+            case 2: return spass<1, S_DRIFT1, true>(c, c->psi, N, N, sr);
+            case 3: return spass<0, S_DRIFT1_SRC, true>(c, c->psi, N, N, sa);
             case 4: {
                 ZArgs za = zargs(c, 0, 0);
-                za.rev = 1;
-                return zpass<Z_INV_SRC>(c, za, c->part_rho);
+                za.ms = 1.0;                         // the source is already m_s |psi|^2 - rho_c
+                return zpass<Z_RSRC>(c, za, nullptr);
             }
-            case 5: return spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sa);
+            case 5: return spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sr);
             case 6: return spass<0, S_GREEN>(c, c->shat, c->hs, N / 2 + 1, sa);
             case 7: return spass<1, S_INV>(c, c->shat, c->hs, N / 2 + 1, sa);
             case 8: return c2r(c, 0, 0, 1);
@@ -156,7 +180,7 @@ struct Ops {
                 int rc;
                 if ((rc = prof_begin(c))) return rc;
                 hipLaunchKernelGGL(k_reduce1, dim3(IFDM_RED_BLOCKS), dim3(256), 0, c->stream, c->part_rho,
-                                   c->nz_blocks, c->part_phi, c->nz_blocks, c->red_stage);
+                                   c->rho_parts, c->part_phi, c->nz_blocks, c->red_stage);
                 LAUNCHCHK(c);
                 hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, c->stream, c->red_stage, IFDM_RED_BLOCKS,
                                    c->red);
@@ -197,6 +221,7 @@ struct Ops {
     static int power_spectrum(ifdm_ctx *c, int nbins, double *dev_out) {
         const SArgs sa = sargs(c, 0.0);
         int rc;
+        c->rho_parts = c->nz_blocks;
         if ((rc = zpass<Z_SRC>(c, zargs(c, 0, 0), c->part_rho))) return rc;
         if ((rc = spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sa))) return rc;
         if ((rc = spass<0, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sa))) return rc;
@@ -212,59 +237,16 @@ struct Ops {
 
     static int poisson(ifdm_ctx *c) {
         int rc;
+        c->rho_parts = c->nz_blocks;
         if ((rc = zpass<Z_SRC>(c, zargs(c, 0, 0), c->part_rho))) return rc;
         for (int i = 5; i <= 9; ++i)
             if ((rc = pass(c, i, 0, 0, 0))) return rc;
         return 0;
     }
 
-    // The x-slab-local chains run slab by slab so that the second pass of a
-    // chain reads what the first just wrote while it is still in the
-    // memory-side cache: (A, B) ... C ... (D, E, F) ... G ... (H, I).
-    static int kick_drift_poisson_slabs(ifdm_ctx *c, double aphi, double arho, double dKE) {
-        const int S = c->slab;
-        const SArgs sa = sargs(c, dKE);
-        ZArgs z0 = zargs(c, 0, 0);
-        ZArgs za = zargs(c, aphi, arho);
-        za.dtab = c->dtab;
-        za.ke_dke = c->ke_coef * dKE;
-        const bool kick = aphi != 0.0 || arho != 0.0;
-        int rc;
-        for (int x0 = 0; x0 < N; x0 += S) {
-            const int nx = std::min(S, N - x0);
-            c->cur_pass = 0;
-            rc = kick ? zpass<Z_KICK_FWD>(c, za, nullptr, x0, nx) : zpass<Z_FWD>(c, za, nullptr, x0, nx);
-            c->cur_pass = 1;
-            if (rc || (rc = spass<1, S_FWD, true>(c, c->psi, N, N, sa, x0, nx))) return rc;
-        }
-        c->cur_pass = 2;
-        if ((rc = spass<0, S_DRIFT, true>(c, c->psi, N, N, sa))) return rc;
-        for (int x0 = 0; x0 < N; x0 += S) {
-            const int nx = std::min(S, N - x0);
-            c->cur_pass = 3;
-            if ((rc = spass<1, S_INV, true>(c, c->psi, N, N, sa, x0, nx))) return rc;
-            c->cur_pass = 4;
-            if ((rc = zpass<Z_INV_SRC>(c, z0, c->part_rho, x0, nx))) return rc;
-            c->cur_pass = 5;
-            if ((rc = spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sa, x0, nx))) return rc;
-        }
-        c->cur_pass = 6;
-        if ((rc = spass<0, S_GREEN>(c, c->shat, c->hs, N / 2 + 1, sa))) return rc;
-        for (int x0 = 0; x0 < N; x0 += S) {
-            const int nx = std::min(S, N - x0);
-            c->cur_pass = 7;
-            if ((rc = spass<1, S_INV>(c, c->shat, c->hs, N / 2 + 1, sa, x0, nx))) return rc;
-            c->cur_pass = 8;
-            if ((rc = c2r(c, x0, nx))) return rc;
-        }
-        return pass(c, 9, 0, 0, 0);
-    }
-
-    // passes A..I + reduction: kick (aphi, arho), drift dKE, Poisson.
+    // passes T, A..I + reduction: kick (aphi, arho), drift dKE, Poisson.
     // With profiling on, each pass is bracketed by events on the stream.
     static int kick_drift_poisson(ifdm_ctx *c, double aphi, double arho, double dKE) {
-        if (c->slab > 0 && c->slab < N && ((long)c->slab * N) % z_lines<N>() == 0)
-            return kick_drift_poisson_slabs(c, aphi, arho, dKE);
         int rc;
         for (int i = 0; i < IFDM_NPASS; ++i)
             if ((rc = pass(c, i, aphi, arho, dKE))) return rc;
@@ -284,14 +266,14 @@ struct Ops {
         return zpass<Z_INV>(c, z0, nullptr);
     }
 
+    // the drift alone: table, then the z, y and x round trips
     static int drift(ifdm_ctx *c, double dKE) {
         SArgs sa = sargs(c, dKE);
         int rc;
         if ((rc = pass(c, 0, 0.0, 0.0, dKE))) return rc;
-        if ((rc = spass<1, S_FWD, true>(c, c->psi, N, N, sa))) return rc;
-        if ((rc = spass<0, S_DRIFT, true>(c, c->psi, N, N, sa))) return rc;
-        if ((rc = spass<1, S_INV, true>(c, c->psi, N, N, sa))) return rc;
-        return zpass<Z_INV>(c, zargs(c, 0, 0), nullptr);
+        if ((rc = pass(c, 1, 0.0, 0.0, dKE))) return rc;
+        if ((rc = spass<1, S_DRIFT1, true>(c, c->psi, N, N, sa))) return rc;
+        return spass<0, S_DRIFT1, true>(c, c->psi, N, N, sa);
     }
 
     static int fft3d(ifdm_ctx *c, int sign) {
@@ -482,7 +464,8 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
         {&c->shat, cs * (size_t)n * n * c->hs},
         {&c->tw, cs * (size_t)n},
         {(void **)&c->k2, ds * (size_t)n},
-        {(void **)&c->part_rho, 2 * ds * (size_t)c->nz_blocks},
+        // density partials: one pair per z-pass block or per x-pass tile
+        {(void **)&c->part_rho, 2 * ds * (size_t)std::max(c->nz_blocks, n * n)},
         {(void **)&c->part_phi, ds * (size_t)c->nz_blocks},
         {(void **)&c->red, 3 * ds},
         {(void **)&c->red_stage, 3 * ds * IFDM_RED_BLOCKS},
@@ -980,11 +963,6 @@ int ifdm_stream_copy(ifdm_ctx *c, size_t bytes, int iters, double *gbs) {
 
 int ifdm_set_option(ifdm_ctx *c, const char *name, long value) {
     if (!c || !name) return IFDM_E_ARG;
-    if (std::strcmp(name, "slab_planes") == 0) {
-        if (value < 0 || value > c->n) { c->err = "slab_planes out of range"; return IFDM_E_ARG; }
-        c->slab = (int)value;
-        return IFDM_OK;
-    }
     if (std::strcmp(name, "persistent") == 0) {
         if (value < 0 || value > 3) { c->err = "persistent must be a mask in 0..3"; return IFDM_E_ARG; }
         c->persistent = (int)value;

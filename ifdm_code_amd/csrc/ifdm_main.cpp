@@ -14,7 +14,6 @@
 //   --ref-ghosts             deposit the x-sorted chunks' trailing particles twice, as at one rank
 //   --nchunks K              IC files per snapshot (default 4)
 //   --device D               HIP device
-//   --slab S                 x-planes per slab chain of the launch schedule
 // Uses only the C ABI of libifdm.so.
 #include <cmath>
 #include <cstdio>
@@ -29,7 +28,7 @@ static void usage() {
     std::fprintf(stderr,
                  "usage: ifdm mfdm g xmax res t_write t_stop ti opath icpath rerun "
                  "[--precision fp64|mixed] [--no-a-triggers] [--variant stan|mpi] [--ref-maxphi-range] "
-                 "[--ref-ghosts] [--nchunks K] [--device D] [--slab S]\n");
+                 "[--ref-ghosts] [--nchunks K] [--device D]\n");
 }
 
 static int check(int rc, ifdm_ctx *ctx, const char *what) {
@@ -53,7 +52,6 @@ int main(int argc, char **argv) {
     const std::string opath = argv[8], icpath = argv[9];
     const bool rerun = std::atof(argv[10]) == 1;
     int precision = IFDM_FP64, a_triggers = 1, variant = 0, ref_range = 0, ref_ghosts = 0, nchunks = 4, device = 0;
-    long slab = 0;
     for (int i = 11; i < argc; ++i) {
         const std::string o = argv[i];
         auto next = [&](void) -> const char * {
@@ -67,7 +65,6 @@ int main(int argc, char **argv) {
         else if (o == "--ref-ghosts") ref_ghosts = 1;
         else if (o == "--nchunks") nchunks = std::atoi(next());
         else if (o == "--device") device = std::atoi(next());
-        else if (o == "--slab") slab = std::atol(next());
         else { usage(); return 2; }
     }
     // argv unit conversions, src/CosmoBEC_MPI_stan.cpp:329-332
@@ -86,7 +83,6 @@ int main(int argc, char **argv) {
                 mfdm, g_in, cfg.xmax, 2 * cfg.xmax, res, twrite, t_stop, ti, opath.c_str());
     ifdm_ctx *ctx = nullptr;
     if (check(ifdm_create(&cfg, &ctx), nullptr, "ifdm_create")) return 1;
-    if (slab && check(ifdm_set_option(ctx, "slab_planes", slab), ctx, "ifdm_set_option")) return 1;
     const time_t t0 = time(nullptr);
     double a = 0.0;
     int rc = 0;

@@ -39,15 +39,13 @@ struct ZArgs {
     double rhoc;      // Poisson source: subtracted density (rho_c; the FP64
                       // mean m_s <|psi|^2> in mixed mode)
     int ref_range;    // max|phi| over the reference's one-rank scan range
-    int blk0;         // first block of this launch (a launch may cover one x-slab)
-    void *dtab;       // drift table to fill (block 0 of a forward z pass), or null
-    const double *k2; // k_i^2 for the drift table
-    double ke_dke;    // ke_coef * dKE for the drift table
+    int blk0;         // first block of this launch
+    void *dtab;       // drift table E[k_z] (the z drift of pass A)
     int rev;          // blocks walk the lines in descending order
 };
 
 struct SArgs {
-    int other0;        // first line index of the fixed axis in this launch (x-slab)
+    int other0;        // first line index of the fixed axis in this launch
     const void *dtab;  // drift table E[i] (pass C)
     const double *k2;  // k_i^2, i < N (src/struct3D.h:76-85 squared)
     double ke_coef;    // 0.5 hbar / (m / c^2)   (src/struct3D.h:121)
@@ -55,6 +53,13 @@ struct SArgs {
     double green;      // -4 pi G
     double fftnorm;    // 1 / N^3
     int rev;           // blocks walk the fixed axis in descending order
+    // pass C's Poisson source (S_DRIFT1_SRC): s = ms |psi|^2 - rhoc into
+    // src (a real field of psi's shape), block max |psi|^2 and sum |psi|^2
+    // into part[2 b], part[2 b + 1] (b = the work-group's linear index)
+    double ms;
+    double rhoc;
+    void *src;
+    double *part;
 };
 
 // Block order.  A pass that walks its lines opposite to the pass before it
@@ -64,8 +69,15 @@ __device__ __forceinline__ int block_order(int rev, unsigned b, unsigned nb) {
     return rev ? (int)(nb - 1 - b) : (int)b;
 }
 
-enum ZMode { Z_FWD = 0, Z_INV = 1, Z_KICK_FWD = 2, Z_INV_SRC = 3, Z_SRC = 4, Z_C2R = 5, Z_RSRC = 6 };
-enum SMode { S_FWD = 0, S_INV = 1, S_DRIFT = 2, S_GREEN = 3 };
+// Z_KICK_DRIFT / Z_DRIFT: (kick,) forward, E[k_z] / N, inverse; Z_SRC: the
+// Poisson source of psi and its real-to-complex transform; Z_RSRC: the
+// same transform of a given real field; Z_C2R: complex-to-real (small N)
+enum ZMode { Z_FWD = 0, Z_INV = 1, Z_SRC = 2, Z_C2R = 3, Z_RSRC = 4, Z_KICK_DRIFT = 5, Z_DRIFT = 6 };
+// S_DRIFT1: forward, the line axis's drift factor E[i] / N, inverse (the
+// drift factorises by axis, so each axis runs its own round trip);
+// S_DRIFT1_SRC: the same, then the Poisson source; S_GREEN: forward,
+// Green's function, inverse
+enum SMode { S_FWD = 0, S_INV = 1, S_GREEN = 2, S_DRIFT1 = 3, S_DRIFT1_SRC = 4 };
 
 template <int N> constexpr int z_lines() {
     constexpr int P = N / Sched<N>::E;
@@ -95,6 +107,12 @@ template <typename T> __device__ __forceinline__ cx<T> expmi(double th) {
 template <typename T, int N>
 __device__ __forceinline__ void write_drift_table(cx<T> *tab, const double *k2, double ke_dke, int tid, int nt) {
     for (int i = tid; i < N; i += nt) tab[i] = expmi<T>((ke_dke * k2[i]));
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_drift_table(cx<T> *__restrict__ tab, const double *__restrict__ k2,
+                                                     double ke_dke) {
+    write_drift_table<T, N>(tab, k2, ke_dke, threadIdx.x, 256);
 }
 
 template <bool MAX> __device__ __forceinline__ double red_op(double a, double b) { return MAX ? fmax(a, b) : a + b; }
@@ -180,7 +198,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = valid ? row[t + e * P] : mkc<T>(0, 0);
 
-        if constexpr (MODE == Z_KICK_FWD) {
+        if constexpr (MODE == Z_KICK_DRIFT) {
             const T *prow = phi + line * N;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -189,8 +207,20 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
                 v[e] = cmul(v[e], expmi<T>(za.aphi * f + za.arho * rr));
             }
         }
-        if constexpr (MODE == Z_FWD || MODE == Z_KICK_FWD) {
-            if (za.dtab && blk == 0) write_drift_table<T, N>((cx<T> *)za.dtab, za.k2, za.ke_dke, tid, NT);
+        if constexpr (MODE == Z_KICK_DRIFT || MODE == Z_DRIFT) {
+            // the z factor of the drift: forward, E[k_z] / N, inverse
+            const cx<T> *tab = (const cx<T> *)za.dtab;
+            fft_line<T, N, -1, wave_line<N>()>(v, L, t, tb);
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = cscale(cmul(v[e], tab[t + e * P]), (T)(1.0 / N));
+            fft_line<T, N, +1, wave_line<N>()>(v, L, t, tb);
+            if (valid) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
+            }
+            return;
+        }
+        if constexpr (MODE == Z_FWD) {
             fft_line<T, N, -1, wave_line<N>()>(v, L, t, tb);
             if (valid) {
 #pragma unroll
@@ -198,14 +228,14 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             }
             return;
         }
-        if constexpr (MODE == Z_INV || MODE == Z_INV_SRC) {
+        if constexpr (MODE == Z_INV) {
             fft_line<T, N, +1, wave_line<N>()>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
             }
         }
-        if constexpr (MODE == Z_INV_SRC || MODE == Z_SRC) {
+        if constexpr (MODE == Z_SRC) {
             double mx = 0.0, sm = 0.0;
             cx<T> s[E];
 #pragma unroll
@@ -316,10 +346,9 @@ __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const T
     else {
         fft_line<T, N, -1, false, FRESH>(v, L, t, tw);
         // spectral indices of this thread's points: along the line i, fixed `other` and k
-        if constexpr (MODE == S_DRIFT) {
-            const cx<T> w = cscale(cmul(tab[other], tab[valid ? k : 0]), (T)sa.fftnorm);
+        if constexpr (MODE == S_DRIFT1 || MODE == S_DRIFT1_SRC) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = cmul(v[e], cmul(tab[t + e * P], w));
+            for (int e = 0; e < E; ++e) v[e] = cscale(cmul(v[e], tab[t + e * P]), (T)(1.0 / N));
         } else {
             const double k2o = k2[other], k2k = valid ? k2[k] : 0.0;
 #pragma unroll
@@ -386,6 +415,26 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
 #pragma unroll
         for (int e = 0; e < E; ++e) io.store(c + (long)(t + e * P) * stride, v[e]);
     }
+    if constexpr (MODE == S_DRIFT1_SRC) {
+        // psi is back in real space: the Poisson source and the step's
+        // density reductions (max and sum of |psi|^2 in FP64)
+        double mx = 0.0, sm = 0.0;
+        T *src = (T *)sa.src + base0;
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
+                mx = fmax(mx, rr);
+                sm += rr;
+                src[c + (long)(t + e * P) * stride] = (T)(sa.ms * rr - sa.rhoc);
+            }
+        }
+        double *sb = reinterpret_cast<double *>(lds);
+        const int nt = TC * P, b = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+        const double bm = block_reduce<true>(mx, sb, tid, nt);
+        const double bs = block_reduce<false>(sm, sb, tid, nt);
+        if (tid == 0) { sa.part[2 * b] = bm; sa.part[2 * b + 1] = bs; }
+    }
 }
 
 // Persistent variant: each work-group walks tiles blockIdx.x, +gridDim.x, ...
@@ -405,13 +454,15 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__
     // value would also wait for the prefetch)
     // (up to N = 512; the 1024-point exchange buffer leaves no room)
     constexpr bool LT = N <= 512;
-    __shared__ cx<T> tabL[(LT && MODE == S_DRIFT) ? N : 1];
+    constexpr bool DR = MODE == S_DRIFT1;
+    static_assert(MODE != S_DRIFT1_SRC, "the source pass runs one tile per work-group");
+    __shared__ cx<T> tabL[(LT && DR) ? N : 1];
     __shared__ double k2L[(LT && MODE == S_GREEN) ? N : 1];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     cx<T> *L = lds + c * LS;
     int tile = blockIdx.x;
     if (tile >= ntiles) return;
-    if constexpr (LT && MODE == S_DRIFT) {
+    if constexpr (LT && DR) {
         for (int i = tid; i < N; i += NT) tabL[i] = ((const cx<T> *)sa.dtab)[i];
     }
     if constexpr (LT && MODE == S_GREEN) {

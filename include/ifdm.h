@@ -129,16 +129,15 @@ int ifdm_run(ifdm_ctx *ctx, const ifdm_dt *sched, int nsteps, ifdm_stats *stats_
 int ifdm_simulate(ifdm_ctx *ctx, double a0, int nsteps, double *a_out, ifdm_dt *sched_out);
 
 /* Times each pass of one step with HIP events on the context's stream,
- * averaged over iters repetitions (ms_out[IFDM_NPASS]; passes A..I of the
- * fused schedule, then the final reduction).  The state is advanced. */
+ * averaged over iters repetitions (ms_out[IFDM_NPASS]: the drift table,
+ * passes A..I of the fused schedule - A, B, C the z, y, x drift round trips
+ * (A with the kick, C with the Poisson source), E the source's real-to-complex
+ * z transform, F..I the Poisson solve - then the final reduction; entry 3 is
+ * pass C and entry 4 pass E).  The state is advanced. */
 #define IFDM_NPASS 10
 int ifdm_time_passes(ifdm_ctx *ctx, const ifdm_dt *dt, int iters, float *ms_out);
 
 /* Launch-schedule options (no effect on results):
- *   "slab_planes" S: 0 (default) launches each pass over the whole grid;
- *   S > 0 runs the x-slab-local pass chains slab by slab, S x-planes per
- *   slab, so that the second pass of a chain finds its input in the
- *   memory-side cache.
  *   "persistent" mask: bit 0 for the y passes, bit 1 for the x passes; a set
  *   bit launches a resident set of work-groups that walk the tiles and
  *   prefetch the next tile's data during the current transform instead of
