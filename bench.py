@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "KDK steps/sec at 512^3: FP64 vs FP32-mixed at stated error; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
-PASS_NAMES = ["T_drift_table", "A_kick_zdrift", "B_ydrift", "C_xdrift_src", "E_src_r2c_z",
+PASS_NAMES = ["T_drift_table", "A_kick_ydrift", "B_xdrift", "C_zdrift_src_r2c",
               "F_yfft_half", "G_xfft_green_xifft_half", "H_yifft_half", "I_c2r_z", "reduce"]
 
 
@@ -36,10 +36,11 @@ def pass_bytes(n: int, sc: int = 16, sr: int = 8) -> list[float]:
     """Algorithmic HBM bytes of each pass: one read and one write of each
     field it touches; s_c complex, s_r real element size; h = (N/2+1)/N.
     A, B, C each run one axis's drift round trip on psi (the drift factor
-    separates by axis), C also writes the real Poisson source."""
+    separates by axis); A also reads phi for the kick, C also writes the
+    source's half spectrum."""
     n3 = float(n) ** 3
     h = (n // 2 + 1) / n
-    return [0.0, n3 * (2 * sc + sr), n3 * 2 * sc, n3 * (2 * sc + sr), n3 * (sr + h * sc),
+    return [0.0, n3 * (2 * sc + sr), n3 * 2 * sc, n3 * (2 * sc + h * sc),
             n3 * 2 * h * sc, n3 * 2 * h * sc, n3 * 2 * h * sc, n3 * (h * sc + sr), 0.0]
 
 
@@ -298,7 +299,7 @@ def counter_traffic(n: int, precision: str) -> dict:
 
 def roofline(n: int, avg: list, sc: int, sr: int, precision: str) -> dict:
     pb = pass_bytes(n, sc, sr)
-    dom = int(np.argmax(avg[:9]))          # the passes; the last entry is the reduction
+    dom = int(np.argmax(avg[:-1]))         # the passes; the last entry is the reduction
     achieved = pb[dom] / (avg[dom] * 1e-3) / 1e9
     traffic = counter_traffic(n, precision).get(PASS_NAMES[dom])
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -22,6 +22,9 @@ _LIB = None
 
 FP64 = 0
 MIXED = 1
+# passes of one step as timed by ifdm_get_profile / ifdm_time_passes
+# (include/ifdm.h, IFDM_NPASS)
+NPASS = 9
 
 
 class IfdmError(RuntimeError):
@@ -389,15 +392,17 @@ class Solver:
         self._chk(lib().ifdm_set_profiling(self._h, int(on)), "ifdm_set_profiling")
 
     def get_profile(self):
-        """(summed ms per pass A..I + reduction, number of profiled steps)"""
-        ms = (ctypes.c_double * 10)()
+        """(summed ms per pass: drift table, A, B, C, F, G, H, I, reduction;
+        number of profiled steps)"""
+        ms = (ctypes.c_double * NPASS)()
         cnt = ctypes.c_long()
         self._chk(lib().ifdm_get_profile(self._h, ms, ctypes.byref(cnt)), "ifdm_get_profile")
         return [float(x) for x in ms], cnt.value
 
     def time_passes(self, dt, iters: int = 3):
-        """Average ms of each pass (A..I, reduction) of one step."""
-        out = (ctypes.c_float * 10)()
+        """Average ms of each pass (drift table, A, B, C, F, G, H, I,
+        reduction) of one step."""
+        out = (ctypes.c_float * NPASS)()
         self._chk(lib().ifdm_time_passes(self._h, ctypes.byref(_Dt(*dt)), iters, out), "ifdm_time_passes")
         return [float(x) for x in out]
 

@@ -48,10 +48,9 @@ struct Ops {
         s.green = c->green;
         s.fftnorm = c->fftnorm;
         s.rev = 0;
-        s.ms = c->ms * c->psc2;
-        s.rhoc = c->mixed ? c->rho_mean : c->cfg.rhoc;
-        s.src = c->phi;
-        s.part = c->part_rho;
+        s.aphi = 0.0;
+        s.arho = 0.0;
+        s.phi = c->phi;
         return s;
     }
 
@@ -101,7 +100,7 @@ struct Ops {
         int rc;
         if ((rc = prof_begin(c))) return rc;
         bool done = false;
-        if constexpr (MODE != S_DRIFT1_SRC) {
+        if constexpr (MODE != S_KICK_DRIFT1) {
           if (c->persistent & (AXIS == 1 ? 1 : 2)) {
             done = true;
             constexpr int TC = s_cols<T, N>();
@@ -123,7 +122,6 @@ struct Ops {
         if (!done) {
             constexpr int TC = s_cols<T, N, WIDE>();
             const int ntz = (kv + TC - 1) / TC;
-            if constexpr (MODE == S_DRIFT1_SRC) c->rho_parts = ntz * no;
             hipLaunchKernelGGL((k_spass<T, N, AXIS, MODE, WIDE>), dim3(ntz, no), dim3(TC * P), 0, c->stream,
                                (cx<T> *)arr, rl, kv, (const cx<T> *)c->tw, sa);
         }
@@ -134,17 +132,19 @@ struct Ops {
     // The drift factor of the step separates by axis (it is exp(-i c k^2
     // dKE) with k^2 = k_x^2 + k_y^2 + k_z^2), so each axis runs its own
     // transform - factor - inverse round trip and psi returns to real space
-    // after three passes instead of five.  One pass of the step by index:
+    // after three passes instead of five; the kick rides on the first of
+    // them and the Poisson source with its z transform on the last.  One
+    // pass of the step by index:
     //   0 T  the per-axis drift table E[i] = exp(-i c k_i^2 dKE)
-    //   1 A  z: kick (aphi, arho; both 0: none), forward, E[k_z]/N, inverse
-    //   2 B  y: forward, E[k_y]/N, inverse
-    //   3 C  x: forward, E[k_x]/N, inverse; Poisson source s = m_s|psi|^2 -
-    //        rho_c into phi's buffer; max and sum of |psi|^2
-    //   4 E  z: real-to-complex transform of s into the half spectrum
-    //   5 F, 6 G, 7 H, 8 I: the Poisson solve on the half spectrum -> phi
-    //   9    final reduction of the partials
-    // B, F and I walk their planes in descending order (the pass before
-    // each of them ascends), starting on what is still in the memory-side cache.
+    //   1 A  y: kick (aphi, arho; both 0: none), forward, E[k_y]/N, inverse
+    //   2 B  x: forward, E[k_x]/N, inverse
+    //   3 C  z: forward, E[k_z]/N, inverse; source s = m_s|psi|^2 - rho_c, its
+    //        real-to-complex transform into the half spectrum; max and sum of
+    //        |psi|^2
+    //   4 F, 5 G, 6 H, 7 I: the Poisson solve on the half spectrum -> phi
+    //   8    final reduction of the partials
+    // F and I walk their planes in descending order (the pass before each
+    // of them ascends), starting on what is still in the memory-side cache.
     static int pass(ifdm_ctx *c, int idx, double aphi, double arho, double dKE) {
         const SArgs sa = sargs(c, dKE);
         SArgs sr = sa;
@@ -160,23 +160,24 @@ struct Ops {
                 return prof_end(c);
             }
             case 1: {
-                ZArgs za = zargs(c, aphi, arho);
-                za.dtab = c->dtab;
-                if (aphi != 0.0 || arho != 0.0) return zpass<Z_KICK_DRIFT>(c, za, nullptr);
-                return zpass<Z_DRIFT>(c, za, nullptr);
+                if (aphi == 0.0 && arho == 0.0) return spass<1, S_DRIFT1, true>(c, c->psi, N, N, sa);
+                SArgs sk = sa;
+                sk.aphi = aphi;
+                sk.arho = arho * c->psc2;
+                return spass<1, S_KICK_DRIFT1, true>(c, c->psi, N, N, sk);
             }
-            case 2: return spass<1, S_DRIFT1, true>(c, c->psi, N, N, sr);
-            case 3: return spass<0, S_DRIFT1_SRC, true>(c, c->psi, N, N, sa);
-            case 4: {
+            case 2: return spass<0, S_DRIFT1, true>(c, c->psi, N, N, sa);
+            case 3: {
                 ZArgs za = zargs(c, 0, 0);
-                za.ms = 1.0;                         // the source is already m_s |psi|^2 - rho_c
-                return zpass<Z_RSRC>(c, za, nullptr);
+                za.dtab = c->dtab;
+                c->rho_parts = c->nz_blocks;
+                return zpass<Z_DRIFT_SRC>(c, za, c->part_rho);
             }
-            case 5: return spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sr);
-            case 6: return spass<0, S_GREEN>(c, c->shat, c->hs, N / 2 + 1, sa);
-            case 7: return spass<1, S_INV>(c, c->shat, c->hs, N / 2 + 1, sa);
-            case 8: return c2r(c, 0, 0, 1);
-            case 9: {
+            case 4: return spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sr);
+            case 5: return spass<0, S_GREEN>(c, c->shat, c->hs, N / 2 + 1, sa);
+            case 6: return spass<1, S_INV>(c, c->shat, c->hs, N / 2 + 1, sa);
+            case 7: return c2r(c, 0, 0, 1);
+            case 8: {
                 int rc;
                 if ((rc = prof_begin(c))) return rc;
                 hipLaunchKernelGGL(k_reduce1, dim3(IFDM_RED_BLOCKS), dim3(256), 0, c->stream, c->part_rho,
@@ -197,7 +198,7 @@ struct Ops {
         c->phi = src;
         int rc = zpass<Z_RSRC>(c, zargs(c, 0, 0), nullptr);
         c->phi = dst;
-        for (int i = 5; i <= 8 && !rc; ++i) rc = pass(c, i, 0, 0, 0);
+        for (int i = 4; i <= 7 && !rc; ++i) rc = pass(c, i, 0, 0, 0);
         c->phi = save;
         return rc;
     }
@@ -231,15 +232,15 @@ struct Ops {
         hipLaunchKernelGGL((k_pk_bin<T, N>), dim3(nb), dim3(256), sizeof(double) * 3 * nbins, c->stream,
                            (const cx<T> *)c->shat, c->hs, nbins, dev_out);
         LAUNCHCHK(c);
-        c->cur_pass = 9;
-        return pass(c, 9, 0, 0, 0);   // reductions (the norm gives the mean density)
+        c->cur_pass = 8;
+        return pass(c, 8, 0, 0, 0);   // reductions (the norm gives the mean density)
     }
 
     static int poisson(ifdm_ctx *c) {
         int rc;
         c->rho_parts = c->nz_blocks;
         if ((rc = zpass<Z_SRC>(c, zargs(c, 0, 0), c->part_rho))) return rc;
-        for (int i = 5; i <= 9; ++i)
+        for (int i = 4; i <= 8; ++i)
             if ((rc = pass(c, i, 0, 0, 0))) return rc;
         return 0;
     }
@@ -266,14 +267,15 @@ struct Ops {
         return zpass<Z_INV>(c, z0, nullptr);
     }
 
-    // the drift alone: table, then the z, y and x round trips
+    // the drift alone: table, then the y, x and z round trips
     static int drift(ifdm_ctx *c, double dKE) {
-        SArgs sa = sargs(c, dKE);
         int rc;
         if ((rc = pass(c, 0, 0.0, 0.0, dKE))) return rc;
         if ((rc = pass(c, 1, 0.0, 0.0, dKE))) return rc;
-        if ((rc = spass<1, S_DRIFT1, true>(c, c->psi, N, N, sa))) return rc;
-        return spass<0, S_DRIFT1, true>(c, c->psi, N, N, sa);
+        if ((rc = pass(c, 2, 0.0, 0.0, dKE))) return rc;
+        ZArgs za = zargs(c, 0, 0);
+        za.dtab = c->dtab;
+        return zpass<Z_DRIFT>(c, za, nullptr);
     }
 
     static int fft3d(ifdm_ctx *c, int sign) {
@@ -464,8 +466,7 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
         {&c->shat, cs * (size_t)n * n * c->hs},
         {&c->tw, cs * (size_t)n},
         {(void **)&c->k2, ds * (size_t)n},
-        // density partials: one pair per z-pass block or per x-pass tile
-        {(void **)&c->part_rho, 2 * ds * (size_t)std::max(c->nz_blocks, n * n)},
+        {(void **)&c->part_rho, 2 * ds * (size_t)c->nz_blocks},
         {(void **)&c->part_phi, ds * (size_t)c->nz_blocks},
         {(void **)&c->red, 3 * ds},
         {(void **)&c->red_stage, 3 * ds * IFDM_RED_BLOCKS},

@@ -53,13 +53,11 @@ struct SArgs {
     double green;      // -4 pi G
     double fftnorm;    // 1 / N^3
     int rev;           // blocks walk the fixed axis in descending order
-    // pass C's Poisson source (S_DRIFT1_SRC): s = ms |psi|^2 - rhoc into
-    // src (a real field of psi's shape), block max |psi|^2 and sum |psi|^2
-    // into part[2 b], part[2 b + 1] (b = the work-group's linear index)
-    double ms;
-    double rhoc;
-    void *src;
-    double *part;
+    // the kick of S_KICK_DRIFT1: phase aphi phi + arho |psi|^2, phi a real
+    // field of psi's shape
+    double aphi;
+    double arho;
+    const void *phi;
 };
 
 // Block order.  A pass that walks its lines opposite to the pass before it
@@ -69,15 +67,17 @@ __device__ __forceinline__ int block_order(int rev, unsigned b, unsigned nb) {
     return rev ? (int)(nb - 1 - b) : (int)b;
 }
 
-// Z_KICK_DRIFT / Z_DRIFT: (kick,) forward, E[k_z] / N, inverse; Z_SRC: the
-// Poisson source of psi and its real-to-complex transform; Z_RSRC: the
-// same transform of a given real field; Z_C2R: complex-to-real (small N)
-enum ZMode { Z_FWD = 0, Z_INV = 1, Z_SRC = 2, Z_C2R = 3, Z_RSRC = 4, Z_KICK_DRIFT = 5, Z_DRIFT = 6 };
+// Z_DRIFT: forward, E[k_z] / N, inverse; Z_SRC: the Poisson source of psi
+// (m_s |psi|^2 - rho_c), its real-to-complex transform and the density
+// reductions; Z_DRIFT_SRC: Z_DRIFT, then Z_SRC on the result; Z_RSRC: the
+// real-to-complex transform of a given real field; Z_C2R: complex-to-real
+// (small N)
+enum ZMode { Z_FWD = 0, Z_INV = 1, Z_SRC = 2, Z_C2R = 3, Z_RSRC = 4, Z_DRIFT = 5, Z_DRIFT_SRC = 6 };
 // S_DRIFT1: forward, the line axis's drift factor E[i] / N, inverse (the
 // drift factorises by axis, so each axis runs its own round trip);
-// S_DRIFT1_SRC: the same, then the Poisson source; S_GREEN: forward,
-// Green's function, inverse
-enum SMode { S_FWD = 0, S_INV = 1, S_GREEN = 2, S_DRIFT1 = 3, S_DRIFT1_SRC = 4 };
+// S_KICK_DRIFT1: the kick exp(-i (aphi phi + arho |psi|^2)) first;
+// S_GREEN: forward, Green's function, inverse
+enum SMode { S_FWD = 0, S_INV = 1, S_GREEN = 2, S_DRIFT1 = 3, S_KICK_DRIFT1 = 4 };
 
 template <int N> constexpr int z_lines() {
     constexpr int P = N / Sched<N>::E;
@@ -198,16 +198,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = valid ? row[t + e * P] : mkc<T>(0, 0);
 
-        if constexpr (MODE == Z_KICK_DRIFT) {
-            const T *prow = phi + line * N;
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const double f = valid ? (double)prow[t + e * P] : 0.0;
-                const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
-                v[e] = cmul(v[e], expmi<T>(za.aphi * f + za.arho * rr));
-            }
-        }
-        if constexpr (MODE == Z_KICK_DRIFT || MODE == Z_DRIFT) {
+        if constexpr (MODE == Z_DRIFT || MODE == Z_DRIFT_SRC) {
             // the z factor of the drift: forward, E[k_z] / N, inverse
             const cx<T> *tab = (const cx<T> *)za.dtab;
             fft_line<T, N, -1, wave_line<N>()>(v, L, t, tb);
@@ -218,7 +209,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
             }
-            return;
+            if constexpr (MODE == Z_DRIFT) return;
         }
         if constexpr (MODE == Z_FWD) {
             fft_line<T, N, -1, wave_line<N>()>(v, L, t, tb);
@@ -235,22 +226,22 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
             }
         }
-        if constexpr (MODE == Z_SRC) {
+        if constexpr (MODE == Z_SRC || MODE == Z_DRIFT_SRC) {
+            // the source replaces psi in the registers (psi is stored already)
             double mx = 0.0, sm = 0.0;
-            cx<T> s[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
                 if (valid) { mx = fmax(mx, rr); sm += rr; }
-                s[e] = mkc<T>(valid ? (T)(za.ms * rr - za.rhoc) : (T)0, (T)0);
+                v[e] = mkc<T>(valid ? (T)(za.ms * rr - za.rhoc) : (T)0, (T)0);
             }
-            fft_line<T, N, -1, wave_line<N>()>(s, L, t, tb);
+            fft_line<T, N, -1, wave_line<N>()>(v, L, t, tb);
             if (valid) {
                 cx<T> *srow = shat + line * HS;
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     const int p = t + e * P;
-                    if (p <= N / 2) srow[p] = s[e];
+                    if (p <= N / 2) srow[p] = v[e];
                 }
             }
             double *sb = reinterpret_cast<double *>(lds);
@@ -346,7 +337,7 @@ __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const T
     else {
         fft_line<T, N, -1, false, FRESH>(v, L, t, tw);
         // spectral indices of this thread's points: along the line i, fixed `other` and k
-        if constexpr (MODE == S_DRIFT1 || MODE == S_DRIFT1_SRC) {
+        if constexpr (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1) {
 #pragma unroll
             for (int e = 0; e < E; ++e) v[e] = cscale(cmul(v[e], tab[t + e * P]), (T)(1.0 / N));
         } else {
@@ -410,30 +401,19 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = valid ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
+    if constexpr (MODE == S_KICK_DRIFT1) {
+        const T *ph = (const T *)sa.phi + base0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const double f = valid ? (double)ph[c + (long)(t + e * P) * stride] : 0.0;
+            const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
+            v[e] = cmul(v[e], expmi<T>(sa.aphi * f + sa.arho * rr));
+        }
+    }
     spass_compute<T, N, MODE, true>(v, L, t, tb, sa, other, k, valid, (const cx<T> *)sa.dtab, sa.k2);
     if (valid) {
 #pragma unroll
         for (int e = 0; e < E; ++e) io.store(c + (long)(t + e * P) * stride, v[e]);
-    }
-    if constexpr (MODE == S_DRIFT1_SRC) {
-        // psi is back in real space: the Poisson source and the step's
-        // density reductions (max and sum of |psi|^2 in FP64)
-        double mx = 0.0, sm = 0.0;
-        T *src = (T *)sa.src + base0;
-        if (valid) {
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
-                mx = fmax(mx, rr);
-                sm += rr;
-                src[c + (long)(t + e * P) * stride] = (T)(sa.ms * rr - sa.rhoc);
-            }
-        }
-        double *sb = reinterpret_cast<double *>(lds);
-        const int nt = TC * P, b = (int)(blockIdx.y * gridDim.x + blockIdx.x);
-        const double bm = block_reduce<true>(mx, sb, tid, nt);
-        const double bs = block_reduce<false>(sm, sb, tid, nt);
-        if (tid == 0) { sa.part[2 * b] = bm; sa.part[2 * b + 1] = bs; }
     }
 }
 
@@ -455,7 +435,7 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__
     // (up to N = 512; the 1024-point exchange buffer leaves no room)
     constexpr bool LT = N <= 512;
     constexpr bool DR = MODE == S_DRIFT1;
-    static_assert(MODE != S_DRIFT1_SRC, "the source pass runs one tile per work-group");
+    static_assert(MODE != S_KICK_DRIFT1, "the kick pass runs one tile per work-group");
     __shared__ cx<T> tabL[(LT && DR) ? N : 1];
     __shared__ double k2L[(LT && MODE == S_GREEN) ? N : 1];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;

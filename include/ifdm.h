@@ -129,12 +129,11 @@ int ifdm_run(ifdm_ctx *ctx, const ifdm_dt *sched, int nsteps, ifdm_stats *stats_
 int ifdm_simulate(ifdm_ctx *ctx, double a0, int nsteps, double *a_out, ifdm_dt *sched_out);
 
 /* Times each pass of one step with HIP events on the context's stream,
- * averaged over iters repetitions (ms_out[IFDM_NPASS]: the drift table,
- * passes A..I of the fused schedule - A, B, C the z, y, x drift round trips
- * (A with the kick, C with the Poisson source), E the source's real-to-complex
- * z transform, F..I the Poisson solve - then the final reduction; entry 3 is
- * pass C and entry 4 pass E).  The state is advanced. */
-#define IFDM_NPASS 10
+ * averaged over iters repetitions (ms_out[IFDM_NPASS]: the drift table; the
+ * passes A, B, C - the y, x, z drift round trips, A with the kick, C with
+ * the Poisson source and its z transform; F, G, H, I - the Poisson solve on
+ * the half spectrum; then the final reduction).  The state is advanced. */
+#define IFDM_NPASS 9
 int ifdm_time_passes(ifdm_ctx *ctx, const ifdm_dt *dt, int iters, float *ms_out);
 
 /* Launch-schedule options (no effect on results):
