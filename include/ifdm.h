@@ -136,11 +136,13 @@ int ifdm_simulate(ifdm_ctx *ctx, double a0, int nsteps, double *a_out, ifdm_dt *
 #define IFDM_NPASS 9
 int ifdm_time_passes(ifdm_ctx *ctx, const ifdm_dt *dt, int iters, float *ms_out);
 
-/* Launch-schedule options (no effect on results):
- *   "persistent" mask: bit 0 for the y passes, bit 1 for the x passes; a set
- *   bit launches a resident set of work-groups that walk the tiles and
- *   prefetch the next tile's data during the current transform instead of
- *   one work-group per tile. */
+/* Launch-schedule options (no effect on results beyond round-off):
+ *   "persistent" mask: bit 0 the y passes on psi (drift only; the kick pass
+ *   always runs one tile per work-group), bit 1 the x passes on psi, bit 2
+ *   the y passes on the half spectrum, bit 3 its x pass; a set bit launches
+ *   a resident set of work-groups that walk the tiles and prefetch the next
+ *   tile's data during the current transform instead of one work-group per
+ *   tile.  Default: 12 (the half-spectrum passes) in FP64, 0 in mixed. */
 int ifdm_set_option(ifdm_ctx *ctx, const char *name, long value);
 
 /* Live per-pass timing: with on != 0 every step run by ifdm_run or
