@@ -53,7 +53,7 @@ struct ifdm_ctx {
     void *ck = nullptr;
     int ck_k = 0, ck_count = 0, ck_nsteps = -1;
     // strided passes with persistent prefetching work-groups: bits 0, 1 the
-    // y, x passes on psi; bits 2, 3 the y, x passes on the half spectrum
+    // y, x passes on psi; bits 2, 3, 4 the passes F, G, H on the half spectrum
     int persistent = 0;
     int num_cu = 0;
     // optional live per-pass timing (ifdm_set_profiling)

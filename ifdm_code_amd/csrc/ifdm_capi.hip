@@ -101,7 +101,8 @@ struct Ops {
         if ((rc = prof_begin(c))) return rc;
         bool done = false;
         if constexpr (MODE != S_KICK_DRIFT1) {
-          if (c->persistent & ((AXIS == 1 ? 1 : 2) << (WIDE ? 0 : 2))) {
+          constexpr int bit = WIDE ? (AXIS == 1 ? 1 : 2) : (AXIS == 0 ? 8 : (MODE == S_INV ? 16 : 4));
+          if (c->persistent & bit) {
             done = true;
             constexpr int TC = s_cols<T, N>();
             const int ntz = (kv + TC - 1) / TC;
@@ -443,8 +444,8 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     c->n3 = (long)cfg->n * cfg->n * cfg->n;
     c->hs = hs_of(cfg->n);
     c->mixed = cfg->precision == IFDM_MIXED;
-    // persistent work-groups measured faster on the FP64 half-spectrum passes only
-    c->persistent = c->mixed ? 0 : 12;
+    // persistent work-groups measured faster on the FP64 passes G and H only
+    c->persistent = c->mixed ? 0 : 24;
     const int n = c->n;
     auto fail = [&](int code, const std::string &m) {
         g_create_err = m;
@@ -967,7 +968,7 @@ int ifdm_stream_copy(ifdm_ctx *c, size_t bytes, int iters, double *gbs) {
 int ifdm_set_option(ifdm_ctx *c, const char *name, long value) {
     if (!c || !name) return IFDM_E_ARG;
     if (std::strcmp(name, "persistent") == 0) {
-        if (value < 0 || value > 15) { c->err = "persistent must be a mask in 0..15"; return IFDM_E_ARG; }
+        if (value < 0 || value > 31) { c->err = "persistent must be a mask in 0..31"; return IFDM_E_ARG; }
         c->persistent = (int)value;
         return IFDM_OK;
     }

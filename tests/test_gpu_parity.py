@@ -260,7 +260,7 @@ def test_mixed_run_density_error(n, steps):
 @pytest.mark.parametrize("n,prec", [(32, 0), (64, 0), (64, 1)])
 def test_persistent_schedule_is_identical(n, prec):
     """Persistent strided passes (resident work-groups that prefetch the next
-    tile; option "persistent" = 15, every strided pass that supports it) run
+    tile; option "persistent" = 31, every strided pass that supports it) run
     the same line transforms as the one-tile-per-work-group launches
     (option 0): the results agree to round-off."""
     F, O = _mods()
@@ -268,7 +268,7 @@ def test_persistent_schedule_is_identical(n, prec):
     cfg = F.Config.from_cli(2.5e-24, G_IN, 5.0, n, precision=prec)
     psi = make_psi(n, 5, cfg.mfdm_solar)
     outs = []
-    for mask in (0, 15):
+    for mask in (0, 31):
         with F.Solver(cfg) as s:
             s.set_option("persistent", mask)
             s.set_psi(psi)
