@@ -100,7 +100,8 @@ struct Ops {
         int rc;
         if ((rc = prof_begin(c))) return rc;
         bool done = false;
-        if constexpr (MODE != S_KICK_DRIFT1) {
+        // (persistent work-groups spill registers at N = 1024 in FP64: up to N = 512)
+        if constexpr (MODE != S_KICK_DRIFT1 && N <= 512) {
           constexpr int bit = WIDE ? (AXIS == 1 ? 1 : 2) : (AXIS == 0 ? 8 : (MODE == S_INV ? 16 : 4));
           if (c->persistent & bit) {
             done = true;
