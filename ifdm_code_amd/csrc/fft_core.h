@@ -201,17 +201,7 @@ __device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const TwBas
 }
 
 template <typename T, int N, int DIR, bool WAVE = false, bool FRESH = false>
-__device__ __forceinline__ void fft_line(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &b0) {
-    // In a kernel with several transforms the compiler would form each
-    // stage's twiddle powers once and keep them live from the first
-    // transform to the last (about 60 registers in FP64, halving the
-    // occupancy of the three-transform pass).  An empty register constraint
-    // on the base values makes every transform form its own powers.
-    TwBase<T, N> b = b0;
-#pragma unroll
-    for (int S = 1; S < Sched<N>::ns; ++S)
-#pragma unroll
-        for (int q = 0; q < Sched<N>::E / Sched<N>::r[S]; ++q) asm volatile("" : "+v"(b.w[S][q].x), "+v"(b.w[S][q].y));
+__device__ __forceinline__ void fft_line(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &b) {
     fft_stage<T, N, DIR, 0, WAVE, FRESH>(v, L, t, b);
     if constexpr (Sched<N>::ns > 1) fft_stage<T, N, DIR, 1, WAVE>(v, L, t, b);
     if constexpr (Sched<N>::ns > 2) fft_stage<T, N, DIR, 2, WAVE>(v, L, t, b);
