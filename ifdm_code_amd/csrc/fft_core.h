@@ -37,22 +37,26 @@ template <int DIR, typename T> __device__ __forceinline__ cx<T> crot(cx<T> a) {
 }
 
 // ---------------------------------------------------------------- schedules
-// Radix per stage (first stage first); E = points per thread.
+// Radix per stage (first stage first); E = points per thread.  Up to
+// N = 512 a thread holds 8 points (one 64-lane wavefront per 512-point
+// line); at 1024 it holds 16, so a line is still one wavefront (its
+// exchanges stay wavefront-local and a strided tile of 16 complex floats is
+// 1024 threads); a stage of radix R then runs E/R butterflies per thread.
 template <int N> struct Sched;
-#define IFDM_SCHED(NN, NSTG, R0, R1, R2, R3)                               \
+#define IFDM_SCHED(NN, EE, NSTG, R0, R1, R2, R3)                           \
     template <> struct Sched<NN> {                                         \
         static constexpr int ns = NSTG;                                    \
-        static constexpr int E = 8;                                        \
+        static constexpr int E = EE;                                       \
         static constexpr int r[4] = {R0, R1, R2, R3};                      \
     };
-IFDM_SCHED(8, 1, 8, 1, 1, 1)
-IFDM_SCHED(16, 2, 8, 2, 1, 1)
-IFDM_SCHED(32, 2, 8, 4, 1, 1)
-IFDM_SCHED(64, 2, 8, 8, 1, 1)
-IFDM_SCHED(128, 3, 8, 8, 2, 1)
-IFDM_SCHED(256, 3, 8, 8, 4, 1)
-IFDM_SCHED(512, 3, 8, 8, 8, 1)
-IFDM_SCHED(1024, 4, 8, 8, 8, 2)
+IFDM_SCHED(8, 8, 1, 8, 1, 1, 1)
+IFDM_SCHED(16, 8, 2, 8, 2, 1, 1)
+IFDM_SCHED(32, 8, 2, 8, 4, 1, 1)
+IFDM_SCHED(64, 8, 2, 8, 8, 1, 1)
+IFDM_SCHED(128, 8, 3, 8, 8, 2, 1)
+IFDM_SCHED(256, 8, 3, 8, 8, 4, 1)
+IFDM_SCHED(512, 8, 3, 8, 8, 8, 1)
+IFDM_SCHED(1024, 16, 4, 8, 8, 8, 2)
 #undef IFDM_SCHED
 
 template <int N> constexpr int ns_before(int s) {

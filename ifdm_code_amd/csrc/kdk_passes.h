@@ -90,7 +90,11 @@ template <typename T, int N, bool WIDE = false> constexpr int s_cols() {
     constexpr int base = 128 / (int)sizeof(cx<T>);
     constexpr int want = (WIDE && sizeof(cx<T>) == 16) ? 2 * base : base;
     constexpr int cap = 1024 / (N / Sched<N>::E);
-    return want < cap ? want : cap;
+    // and the exchange buffer within the 160 KB of LDS
+    constexpr int lcap = 163840 / (lds_line<N>() * (int)sizeof(cx<T>));
+    int c = want < cap ? want : cap;
+    while (c > lcap) c /= 2;
+    return c;
 }
 
 // exp(-i th) in the storage precision T: the FP64 argument is reduced mod
@@ -258,8 +262,11 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
 //   Z[k] = (X[k] + conj X[N/2-k]) + i W^-k (X[k] - conj X[N/2-k]),
 // then z = IDFT_{N/2}(Z) (unnormalised) gives y[2m] = Re z[m],
 // y[2m+1] = Im z[m] with y the unnormalised length-N inverse of X.
-// Lines per block are twice those of the full-length z passes.
-template <int N> constexpr int c2r_lines() { return 2 * z_lines<N>(); }
+// A block has the thread count of a full-length z-pass block, so it holds
+// (N/E_N) / (N/2 / E_{N/2}) times as many (half-length) lines.
+template <int N> constexpr int c2r_lines() {
+    return z_lines<N>() * (N / Sched<N>::E) / ((N / 2) / Sched<N / 2>::E);
+}
 
 template <typename T, int N>
 __global__ __launch_bounds__(z_lines<N>() * (N / Sched<N>::E))
@@ -313,9 +320,12 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
     double *sb = reinterpret_cast<double *>(lds);
     const double bm = block_reduce<true>(mx, sb, tid, NT);
     if (tid == 0) {
-        // partials are indexed per full-length z-pass block: two per block here
-        part[2 * blk] = bm;
-        part[2 * blk + 1] = 0.0;
+        // partials are indexed per full-length z-pass block: LZ / z_lines of
+        // them per block here
+        constexpr int RATIO = LZ / z_lines<N>();
+        part[RATIO * blk] = bm;
+#pragma unroll
+        for (int i = 1; i < RATIO; ++i) part[RATIO * blk + i] = 0.0;
     }
 }
 
