@@ -37,14 +37,11 @@ template <int DIR, typename T> __device__ __forceinline__ cx<T> crot(cx<T> a) {
 }
 
 // ---------------------------------------------------------------- schedules
-// Radix per stage (first stage first); E = points per thread.  Up to
-// N = 512 a thread holds 8 points (one 64-lane wavefront per 512-point
-// line); at 1024 it holds 16, so a line is still one wavefront (its
-// exchanges stay wavefront-local and a strided tile of 16 complex floats is
-// 1024 threads); a stage of radix R then runs E/R butterflies per thread.
-template <int N> struct Sched;
+// Radix per stage (first stage first); E = points per thread (a stage of
+// radix R runs E/R butterflies per thread).
+template <int N, int E> struct Sched;
 #define IFDM_SCHED(NN, EE, NSTG, R0, R1, R2, R3)                           \
-    template <> struct Sched<NN> {                                         \
+    template <> struct Sched<NN, EE> {                                     \
         static constexpr int ns = NSTG;                                    \
         static constexpr int E = EE;                                       \
         static constexpr int r[4] = {R0, R1, R2, R3};                      \
@@ -56,12 +53,21 @@ IFDM_SCHED(64, 8, 2, 8, 8, 1, 1)
 IFDM_SCHED(128, 8, 3, 8, 8, 2, 1)
 IFDM_SCHED(256, 8, 3, 8, 8, 4, 1)
 IFDM_SCHED(512, 8, 3, 8, 8, 8, 1)
+IFDM_SCHED(1024, 8, 4, 8, 8, 8, 2)
 IFDM_SCHED(1024, 16, 4, 8, 8, 8, 2)
 #undef IFDM_SCHED
 
-template <int N> constexpr int ns_before(int s) {
+// Points per thread: 8 (one 64-lane wavefront per 512-point line), and 16
+// for complex floats at N = 1024, where a line then stays one wavefront and
+// a strided tile of 16 columns (128-byte rows) fits 1024 threads.  Complex
+// doubles keep 8 at 1024: 16 halves their occupancy (measured slower).
+constexpr int pts_of(bool fp32, int n) { return (n == 1024 && fp32) ? 16 : 8; }
+template <typename T, int N> constexpr int pts() { return pts_of(sizeof(T) == 4, N); }
+template <typename T, int N> using SchedT = Sched<N, pts<T, N>()>;
+
+template <int N, int E> constexpr int ns_before(int s) {
     int p = 1;
-    for (int i = 0; i < s; ++i) p *= Sched<N>::r[i];
+    for (int i = 0; i < s; ++i) p *= Sched<N, E>::r[i];
     return p;
 }
 
@@ -113,7 +119,7 @@ template <int R, int DIR, typename T> __device__ __forceinline__ void dft(cx<T> 
 // values are loaded once per thread at the start of a pass, so their
 // latency hides behind the data loads instead of stalling each stage.
 template <typename T, int N> struct TwBase {
-    static constexpr int E = Sched<N>::E;
+    static constexpr int E = SchedT<T, N>::E;
     cx<T> w[4][E];  // [stage][butterfly]; stage 0 unused
 };
 
@@ -122,10 +128,10 @@ template <typename T, int N> struct TwBase {
 // full-length table)
 template <typename T, int N, int TS = 1>
 __device__ __forceinline__ void load_twiddles(TwBase<T, N> &b, int t, const cx<T> *__restrict__ tw) {
-    constexpr int E = Sched<N>::E, P = N / E;
+    constexpr int E = SchedT<T, N>::E, P = N / E;
 #pragma unroll
-    for (int S = 1; S < Sched<N>::ns; ++S) {
-        const int R = Sched<N>::r[S], Q = E / R, NS = ns_before<N>(S);
+    for (int S = 1; S < SchedT<T, N>::ns; ++S) {
+        const int R = SchedT<T, N>::r[S], Q = E / R, NS = ns_before<N, E>(S);
 #pragma unroll
         for (int q = 0; q < Q; ++q) b.w[S][q] = tw[TS * (((t + q * P) & (NS - 1)) * (N / (NS * R)))];
     }
@@ -171,9 +177,9 @@ template <bool WAVE> __device__ __forceinline__ void line_sync() {
 // used its LDS, so the first scatter needs no preceding synchronisation.
 template <typename T, int N, int DIR, int S, bool WAVE = false, bool FRESH = false>
 __device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &b) {
-    constexpr int E = Sched<N>::E, P = N / E, R = Sched<N>::r[S], Q = E / R;
-    constexpr int NS = ns_before<N>(S);
-    constexpr bool FIRST = (S == 0), LAST = (S == Sched<N>::ns - 1);
+    constexpr int E = SchedT<T, N>::E, P = N / E, R = SchedT<T, N>::r[S], Q = E / R;
+    constexpr int NS = ns_before<N, E>(S);
+    constexpr bool FIRST = (S == 0), LAST = (S == SchedT<T, N>::ns - 1);
     cx<T> a[Q][R];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -207,9 +213,9 @@ __device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const TwBas
 template <typename T, int N, int DIR, bool WAVE = false, bool FRESH = false>
 __device__ __forceinline__ void fft_line(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &b) {
     fft_stage<T, N, DIR, 0, WAVE, FRESH>(v, L, t, b);
-    if constexpr (Sched<N>::ns > 1) fft_stage<T, N, DIR, 1, WAVE>(v, L, t, b);
-    if constexpr (Sched<N>::ns > 2) fft_stage<T, N, DIR, 2, WAVE>(v, L, t, b);
-    if constexpr (Sched<N>::ns > 3) fft_stage<T, N, DIR, 3, WAVE>(v, L, t, b);
+    if constexpr (SchedT<T, N>::ns > 1) fft_stage<T, N, DIR, 1, WAVE>(v, L, t, b);
+    if constexpr (SchedT<T, N>::ns > 2) fft_stage<T, N, DIR, 2, WAVE>(v, L, t, b);
+    if constexpr (SchedT<T, N>::ns > 3) fft_stage<T, N, DIR, 3, WAVE>(v, L, t, b);
 }
 
 // convenience: load the base twiddles and transform
@@ -222,6 +228,6 @@ __device__ __forceinline__ void fft_line(cx<T> *v, cx<T> *L, int t, const cx<T> 
 
 // true when a line of N points (N/E consecutive threads) lies inside one
 // wavefront of 64 lanes
-template <int N> constexpr bool wave_line() { return N / Sched<N>::E <= 64; }
+template <typename T, int N> constexpr bool wave_line() { return N / SchedT<T, N>::E <= 64; }
 
 }  // namespace ifdm

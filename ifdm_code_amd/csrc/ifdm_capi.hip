@@ -24,7 +24,7 @@ namespace {
 
 template <typename T, int N>
 struct Ops {
-    static constexpr int E = Sched<N>::E, P = N / E;
+    static constexpr int E = SchedT<T, N>::E, P = N / E;
 
     static ZArgs zargs(ifdm_ctx *c, double aphi, double arho) {
         ZArgs z;
@@ -57,7 +57,7 @@ struct Ops {
     // z pass over x-planes [x0, x0 + nx) (all planes when nx == 0)
     template <int MODE>
     static int zpass(ifdm_ctx *c, ZArgs za, double *part, int x0 = 0, int nx = 0) {
-        constexpr int LZ = z_lines<N>();
+        constexpr int LZ = z_lines<T, N>();
         if (nx == 0) nx = N;
         const long l0 = (long)x0 * N, nl = (long)nx * N;
         za.blk0 = (int)(l0 / LZ);
@@ -76,14 +76,14 @@ struct Ops {
         if constexpr (N < 32) {
             return zpass<Z_C2R>(c, za, c->part_phi, x0, nx);
         } else {
-            constexpr int LZ = c2r_lines<N>();
+            constexpr int LZ = c2r_lines<T, N>();
             if (nx == 0) nx = N;
             const long l0 = (long)x0 * N, nl = (long)nx * N;
             za.blk0 = (int)(l0 / LZ);
             const int nb = (int)((nl + LZ - 1) / LZ);
             int rc;
             if ((rc = prof_begin(c))) return rc;
-            hipLaunchKernelGGL((k_c2r_half<T, N>), dim3(nb), dim3(z_lines<N>() * P), 0, c->stream,
+            hipLaunchKernelGGL((k_c2r_half<T, N>), dim3(nb), dim3(z_lines<T, N>() * P), 0, c->stream,
                                (const cx<T> *)c->shat, (T *)c->phi, (const cx<T> *)c->tw, za, c->part_phi);
             LAUNCHCHK(c);
             return prof_end(c);
@@ -461,8 +461,8 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
         return fail(IFDM_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     // element sizes of the stored fields (reductions stay FP64)
     const size_t rs = c->mixed ? sizeof(float) : sizeof(double), cs = 2 * rs, ds = sizeof(double);
-    // z-pass block count: lines per block = min(256 / (n / 8), n * n)
-    const int per = std::min(256 / (n / 8), n * n);
+    // z-pass block count: lines per block = min(256 / (n / E), n * n)
+    const int per = std::min(256 / (n / pts_of(c->mixed, n)), n * n);
     c->nz_blocks = (n * n + per - 1) / per;
     struct { void **p; size_t bytes; } allocs[] = {
         {&c->psi, cs * (size_t)c->n3},

@@ -79,8 +79,8 @@ enum ZMode { Z_FWD = 0, Z_INV = 1, Z_SRC = 2, Z_C2R = 3, Z_RSRC = 4, Z_DRIFT = 5
 // S_GREEN: forward, Green's function, inverse
 enum SMode { S_FWD = 0, S_INV = 1, S_GREEN = 2, S_DRIFT1 = 3, S_KICK_DRIFT1 = 4 };
 
-template <int N> constexpr int z_lines() {
-    constexpr int P = N / Sched<N>::E;
+template <typename T, int N> constexpr int z_lines() {
+    constexpr int P = N / SchedT<T, N>::E;
     return (256 / P) < (N * N) ? (256 / P) : (N * N);
 }
 // z-columns per strided-pass tile: 128-byte row segments (8 complex
@@ -89,7 +89,7 @@ template <int N> constexpr int z_lines() {
 template <typename T, int N, bool WIDE = false> constexpr int s_cols() {
     constexpr int base = 128 / (int)sizeof(cx<T>);
     constexpr int want = (WIDE && sizeof(cx<T>) == 16) ? 2 * base : base;
-    constexpr int cap = 1024 / (N / Sched<N>::E);
+    constexpr int cap = 1024 / (N / SchedT<T, N>::E);
     // and the exchange buffer within the 160 KB of LDS
     constexpr int lcap = 163840 / (lds_line<N>() * (int)sizeof(cx<T>));
     int c = want < cap ? want : cap;
@@ -138,10 +138,10 @@ __device__ __forceinline__ double block_reduce(double v, double *sbuf, int tid, 
 
 // ------------------------------------------------------------------ z passes
 template <typename T, int N, int MODE>
-__global__ __launch_bounds__(z_lines<N>() * (N / Sched<N>::E))
+__global__ __launch_bounds__((z_lines<T, N>() * (N / SchedT<T, N>::E)))
 void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ shat,
              const cx<T> *__restrict__ tw, ZArgs za, double *__restrict__ part) {
-    constexpr int E = Sched<N>::E, P = N / E, LZ = z_lines<N>(), LS = lds_line<N>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, LZ = z_lines<T, N>(), LS = lds_line<N>();
     constexpr int HS = hs_of(N), NT = LZ * P;
     __shared__ cx<T> lds[LZ * LS];
     const int tid = threadIdx.x, l = tid / P, t = tid - l * P;
@@ -165,7 +165,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             }
             v[e] = s;
         }
-        fft_line<T, N, +1, wave_line<N>()>(v, L, t, tb);
+        fft_line<T, N, +1, wave_line<T, N>()>(v, L, t, tb);
         double mx = 0.0;
         T *out = phi + line * N;
 #pragma unroll
@@ -187,7 +187,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         cx<T> s[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) s[e] = mkc<T>(valid ? (T)(za.ms * (double)rrow[t + e * P]) : (T)0, (T)0);
-        fft_line<T, N, -1, wave_line<N>()>(s, L, t, tb);
+        fft_line<T, N, -1, wave_line<T, N>()>(s, L, t, tb);
         if (valid) {
             cx<T> *srow = shat + line * HS;
 #pragma unroll
@@ -205,10 +205,10 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         if constexpr (MODE == Z_DRIFT || MODE == Z_DRIFT_SRC) {
             // the z factor of the drift: forward, E[k_z] / N, inverse
             const cx<T> *tab = (const cx<T> *)za.dtab;
-            fft_line<T, N, -1, wave_line<N>()>(v, L, t, tb);
+            fft_line<T, N, -1, wave_line<T, N>()>(v, L, t, tb);
 #pragma unroll
             for (int e = 0; e < E; ++e) v[e] = cscale(cmul(v[e], tab[t + e * P]), (T)(1.0 / N));
-            fft_line<T, N, +1, wave_line<N>()>(v, L, t, tb);
+            fft_line<T, N, +1, wave_line<T, N>()>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -216,7 +216,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             if constexpr (MODE == Z_DRIFT) return;
         }
         if constexpr (MODE == Z_FWD) {
-            fft_line<T, N, -1, wave_line<N>()>(v, L, t, tb);
+            fft_line<T, N, -1, wave_line<T, N>()>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -224,7 +224,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             return;
         }
         if constexpr (MODE == Z_INV) {
-            fft_line<T, N, +1, wave_line<N>()>(v, L, t, tb);
+            fft_line<T, N, +1, wave_line<T, N>()>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -239,7 +239,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
                 if (valid) { mx = fmax(mx, rr); sm += rr; }
                 v[e] = mkc<T>(valid ? (T)(za.ms * rr - za.rhoc) : (T)0, (T)0);
             }
-            fft_line<T, N, -1, wave_line<N>()>(v, L, t, tb);
+            fft_line<T, N, -1, wave_line<T, N>()>(v, L, t, tb);
             if (valid) {
                 cx<T> *srow = shat + line * HS;
 #pragma unroll
@@ -264,17 +264,17 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
 // y[2m+1] = Im z[m] with y the unnormalised length-N inverse of X.
 // A block has the thread count of a full-length z-pass block, so it holds
 // (N/E_N) / (N/2 / E_{N/2}) times as many (half-length) lines.
-template <int N> constexpr int c2r_lines() {
-    return z_lines<N>() * (N / Sched<N>::E) / ((N / 2) / Sched<N / 2>::E);
+template <typename T, int N> constexpr int c2r_lines() {
+    return z_lines<T, N>() * (N / SchedT<T, N>::E) / ((N / 2) / SchedT<T, N / 2>::E);
 }
 
 template <typename T, int N>
-__global__ __launch_bounds__(z_lines<N>() * (N / Sched<N>::E))
+__global__ __launch_bounds__((z_lines<T, N>() * (N / SchedT<T, N>::E)))
 void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T> *__restrict__ tw, ZArgs za,
                 double *__restrict__ part) {
-    constexpr int H = N / 2, E = Sched<H>::E, P = H / E, LZ = c2r_lines<N>(), LS = lds_line<H>();
+    constexpr int H = N / 2, E = SchedT<T, H>::E, P = H / E, LZ = c2r_lines<T, N>(), LS = lds_line<H>();
     constexpr int HS = hs_of(N), NT = LZ * P;
-    static_assert(NT == z_lines<N>() * (N / Sched<N>::E), "block shape");
+    static_assert(NT == z_lines<T, N>() * (N / SchedT<T, N>::E), "block shape");
     __shared__ cx<T> lds[(LZ * LS > NT / 2 + 1) ? LZ * LS : NT / 2 + 1];
     const int tid = threadIdx.x, l = tid / P, t = tid - l * P;
     const int blk = za.blk0 + block_order(za.rev, blockIdx.x, gridDim.x);
@@ -302,7 +302,7 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
         dif = cmul(dif, w);
         v[e] = mkc<T>(sum.x - dif.y, sum.y + dif.x);           // sum + i dif
     }
-    fft_line<T, H, +1, wave_line<H>()>(v, L, t, tb);
+    fft_line<T, H, +1, wave_line<T, H>()>(v, L, t, tb);
     double mx = 0.0;
     cx<T> *out = reinterpret_cast<cx<T> *>(phi + line * N);
 #pragma unroll
@@ -322,7 +322,7 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
     if (tid == 0) {
         // partials are indexed per full-length z-pass block: LZ / z_lines of
         // them per block here
-        constexpr int RATIO = LZ / z_lines<N>();
+        constexpr int RATIO = LZ / z_lines<T, N>();
         part[RATIO * blk] = bm;
 #pragma unroll
         for (int i = 1; i < RATIO; ++i) part[RATIO * blk + i] = 0.0;
@@ -341,7 +341,7 @@ template <typename T, int N, int MODE, bool FRESH>
 __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &tw,
                                               const SArgs &sa, int other, int k, bool valid,
                                               const cx<T> *tab, const double *k2) {
-    constexpr int E = Sched<N>::E, P = N / E;
+    constexpr int E = SchedT<T, N>::E, P = N / E;
     if constexpr (MODE == S_FWD) fft_line<T, N, -1, false, FRESH>(v, L, t, tw);
     else if constexpr (MODE == S_INV) fft_line<T, N, +1, false, FRESH>(v, L, t, tw);
     else {
@@ -394,9 +394,9 @@ template <typename T, int N> struct TileIO {
 
 // one tile per work-group: grid (z-column tiles, lines of the fixed axis)
 template <typename T, int N, int AXIS, int MODE, bool WIDE>
-__global__ __launch_bounds__((s_cols<T, N, WIDE>() * (N / Sched<N>::E)))
+__global__ __launch_bounds__((s_cols<T, N, WIDE>() * (N / SchedT<T, N>::E)))
 void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw, SArgs sa) {
-    constexpr int E = Sched<N>::E, P = N / E, TC = s_cols<T, N, WIDE>(), LS = lds_line<N>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE>(), LS = lds_line<N>();
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     const int k0 = blockIdx.x * TC, k = k0 + c;
@@ -433,10 +433,10 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
 // overlap the butterflies and LDS exchanges.  The first LDS write of every
 // transform is preceded by a barrier, so consecutive tiles never race on LDS.
 template <typename T, int N, int AXIS, int MODE>
-__global__ __launch_bounds__((s_cols<T, N>() * (N / Sched<N>::E)))
+__global__ __launch_bounds__((s_cols<T, N>() * (N / SchedT<T, N>::E)))
 void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw, SArgs sa, int ntz,
                 int ntiles) {
-    constexpr int E = Sched<N>::E, P = N / E, TC = s_cols<T, N>(), LS = lds_line<N>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N>(), LS = lds_line<N>();
     constexpr int NT = TC * P;
     __shared__ cx<T> lds[TC * LS];
     // the spectral multiply's table in LDS, so no global load is issued
