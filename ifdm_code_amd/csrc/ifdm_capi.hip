@@ -91,7 +91,7 @@ struct Ops {
     }
 
     // strided pass; for AXIS 1 the fixed index (x) runs over [o0, o0 + no)
-    // WIDE: 256-byte row tiles in FP64 (the psi passes; the half spectrum
+    // WIDE: 256-byte row tiles in FP64 (the x pass on psi; the half spectrum
     // keeps 128-byte tiles, whose last tile holds a single valid column)
     template <int AXIS, int MODE, bool WIDE = false>
     static int spass(ifdm_ctx *c, void *arr, int rl, int kv, SArgs sa, int o0 = 0, int no = 0) {
@@ -102,7 +102,8 @@ struct Ops {
         bool done = false;
         // (persistent work-groups spill registers at N = 1024 in FP64: up to N = 512)
         if constexpr (MODE != S_KICK_DRIFT1 && N <= 512) {
-          constexpr int bit = WIDE ? (AXIS == 1 ? 1 : 2) : (AXIS == 0 ? 8 : (MODE == S_INV ? 16 : 4));
+          constexpr bool PSI = MODE == S_DRIFT1;
+          constexpr int bit = PSI ? (AXIS == 1 ? 1 : 2) : (AXIS == 0 ? 8 : (MODE == S_INV ? 16 : 4));
           if (c->persistent & bit) {
             done = true;
             constexpr int TC = s_cols<T, N>();
@@ -162,11 +163,13 @@ struct Ops {
                 return prof_end(c);
             }
             case 1: {
-                if (aphi == 0.0 && arho == 0.0) return spass<1, S_DRIFT1, true>(c, c->psi, N, N, sa);
+                // 128-byte tiles here (two work-groups per CU in FP64 overlap the
+                // kick's sincos with loads: measured faster than 256-byte tiles)
+                if (aphi == 0.0 && arho == 0.0) return spass<1, S_DRIFT1>(c, c->psi, N, N, sa);
                 SArgs sk = sa;
                 sk.aphi = aphi;
                 sk.arho = arho * c->psc2;
-                return spass<1, S_KICK_DRIFT1, true>(c, c->psi, N, N, sk);
+                return spass<1, S_KICK_DRIFT1>(c, c->psi, N, N, sk);
             }
             case 2: return spass<0, S_DRIFT1, true>(c, c->psi, N, N, sa);
             case 3: {

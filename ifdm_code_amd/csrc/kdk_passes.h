@@ -85,7 +85,7 @@ template <typename T, int N> constexpr int z_lines() {
 }
 // z-columns per strided-pass tile: 128-byte row segments (8 complex
 // doubles, 16 complex floats); WIDE doubles them for FP64 (256-byte rows,
-// used on psi), and a work-group never exceeds 1024 threads
+// used on the x pass over psi), and a work-group never exceeds 1024 threads
 template <typename T, int N, bool WIDE = false> constexpr int s_cols() {
     constexpr int base = 128 / (int)sizeof(cx<T>);
     constexpr int want = (WIDE && sizeof(cx<T>) == 16) ? 2 * base : base;
