@@ -326,6 +326,8 @@ def main():
                     help="also run the adjoint study over this many steps (0: skip)")
     ap.add_argument("--adjoint-k", type=int, default=50, help="checkpoint interval of the adjoint study")
     ap.add_argument("--adjoint-fd", type=int, default=2, help="finite-difference directions of the adjoint study")
+    ap.add_argument("--plane-pad", type=int, default=0,
+                    help="device cells after each x-plane of psi and phi (0: the library's choice, <0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -354,7 +356,7 @@ def main():
     mt = cfg0.mfdm / F.constant("c") ** 2
     g = 0.01 * mt * rms_phi / mean_rho              # eV Mpc^3
     g_in = g * F.constant("cm2Mpc") ** 3            # eV cm^3, the CLI unit
-    cfg = F.Config.from_cli(2.5e-24, g_in, xmax_in, n)
+    cfg = F.Config.from_cli(2.5e-24, g_in, xmax_in, n, plane_pad=args.plane_pad)
     log(f"[bench] g = {g_in:.4e} eV cm^3")
 
     sps, el, avg, sched, psi64, phi64, a = timed_adaptive(F, cfg, psi, args.warmup, args.steps,
@@ -387,7 +389,7 @@ def main():
     }
 
     if not args.no_mixed:
-        cfgm = F.Config.from_cli(2.5e-24, g_in, xmax_in, n, precision=F.MIXED)
+        cfgm = F.Config.from_cli(2.5e-24, g_in, xmax_in, n, precision=F.MIXED, plane_pad=args.plane_pad)
         spsm, elm, avgm, _schm, _p, _f, am = timed_adaptive(F, cfgm, psi, args.warmup, args.steps,
                                                             args.prof_steps, False)
         log(f"[bench] mixed: {args.steps} steps in {elm:.3f} s ({spsm:.2f} steps/s)")

@@ -34,7 +34,7 @@ class IfdmError(RuntimeError):
 class _Config(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("precision", ctypes.c_int), ("xmax", ctypes.c_double),
                 ("mfdm", ctypes.c_double), ("g", ctypes.c_double), ("rhoc", ctypes.c_double),
-                ("ref_maxphi_range", ctypes.c_int), ("device", ctypes.c_int)]
+                ("ref_maxphi_range", ctypes.c_int), ("device", ctypes.c_int), ("plane_pad", ctypes.c_int)]
 
 
 class _Stats(ctypes.Structure):
@@ -166,6 +166,7 @@ class Config:
     precision: int = FP64
     ref_maxphi_range: bool = False
     device: int = 0
+    plane_pad: int = 0     # device cells after each x-plane (0: built-in, <0: none)
 
     @classmethod
     def from_cli(cls, mfdm_ev: float, g_ev_cm3: float, xmax_mpc_h: float, res: int, **kw):
@@ -178,7 +179,7 @@ class Config:
 
     def _c(self) -> _Config:
         return _Config(self.n, self.precision, self.xmax, self.mfdm, self.g, self.rhoc,
-                       int(self.ref_maxphi_range), self.device)
+                       int(self.ref_maxphi_range), self.device, int(self.plane_pad))
 
 
 def _dp(a: np.ndarray):

@@ -12,10 +12,13 @@ struct ifdm_ctx {
     ifdm_config cfg{};
     int n = 0;
     long n3 = 0;
+    // psi, phi and the adjoint fields hold x-plane after x-plane, each plane
+    // N*N contiguous cells, planes ps cells apart (ps = N*N + pad); np = N*ps
+    long ps = 0, np = 0;
     int hs = 0;
     hipStream_t stream = nullptr;
-    void *psi = nullptr;       // N^3 complex
-    void *phi = nullptr;       // N^3 real
+    void *psi = nullptr;       // N^3 complex (plane stride ps)
+    void *phi = nullptr;       // N^3 real (plane stride ps)
     void *shat = nullptr;      // N*N*HS complex
     void *tw = nullptr;        // N complex twiddles
     double *k2 = nullptr;      // N doubles
@@ -78,6 +81,23 @@ static constexpr int IFDM_RED_BLOCKS = 256;
     } while (0)
 
 #define LAUNCHCHK(ctx) HIPCHK(ctx, hipGetLastError())
+
+// Copies nx x-planes between a dense array (planes N*N cells apart) and a
+// device field of plane stride c->ps, on the context's stream; elem = bytes
+// per cell, to_field selects the direction.
+static inline hipError_t plane_copy(const ifdm_ctx *c, void *field, void *dense, size_t elem, int nx, bool to_field,
+                                    hipMemcpyKind kind) {
+    const size_t pb = elem * (size_t)c->n * c->n, fb = elem * (size_t)c->ps;
+    if (fb == pb)
+        return to_field ? hipMemcpyAsync(field, dense, pb * nx, kind, c->stream)
+                        : hipMemcpyAsync(dense, field, pb * nx, kind, c->stream);
+    for (int x = 0; x < nx; ++x) {
+        char *f = (char *)field + fb * x, *d = (char *)dense + pb * x;
+        const hipError_t e = to_field ? hipMemcpyAsync(f, d, pb, kind, c->stream) : hipMemcpyAsync(d, f, pb, kind, c->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
 
 static constexpr int IFDM_EV_POOL = 4096;
 

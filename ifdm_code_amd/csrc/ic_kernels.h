@@ -87,10 +87,11 @@ static __global__ __launch_bounds__(256) void k_ic_velnorm(const double *__restr
 }
 
 // theta source: 0.5 (dvx + dvy + dvz) * unit / dx with periodic central
-// differences (src/gadget_IC.h:190-219 at one rank), as a complex field
+// differences (src/gadget_IC.h:190-219 at one rank), as a complex field of
+// plane stride ps
 static __global__ __launch_bounds__(256) void k_ic_divergence(const double *__restrict__ vx, const double *__restrict__ vy,
                                                        const double *__restrict__ vz, cx<double> *__restrict__ out,
-                                                       int n, double scale) {
+                                                       int n, long ps, double scale) {
     const long n3 = (long)n * n * n;
     for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < n3; o += (long)gridDim.x * blockDim.x) {
         const int k = (int)(o % n), j = (int)((o / n) % n), i = (int)(o / ((long)n * n));
@@ -100,17 +101,20 @@ static __global__ __launch_bounds__(256) void k_ic_divergence(const double *__re
         const double dvx = vx[k + N * (j + ip * N)] - vx[k + N * (j + im * N)];
         const double dvy = vy[k + N * (jp + i * N)] - vy[k + N * (jm + i * N)];
         const double dvz = vz[kp + N * (j + i * N)] - vz[km + N * (j + i * N)];
-        out[o] = mkc<double>(0.5 * (dvx + dvy + dvz) * scale, 0.0);
+        out[i * ps + (o - i * N * N)] = mkc<double>(0.5 * (dvx + dvy + dvz) * scale, 0.0);
     }
 }
 
 // psi = 1.225 sqrt(rho uc / m_s) exp(i cph theta), theta complex
-// (src/gadget_IC.h:666); psi overwrites theta in place
+// (src/gadget_IC.h:666); psi overwrites theta in place (theta and psi with
+// plane stride ps, rho dense)
 static __global__ __launch_bounds__(256) void k_ic_assemble(const double *__restrict__ rho, cx<double> *__restrict__ th,
-                                                     long n3, double uc, double ms, double cph) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
+                                                     int n, long ps, double uc, double ms, double cph) {
+    const long n2 = (long)n * n, n3 = n2 * n;
+    for (long o = (long)blockIdx.x * blockDim.x + threadIdx.x; o < n3; o += (long)gridDim.x * blockDim.x) {
+        const long i = (o / n2) * ps + o % n2;
         const cx<double> t = th[i];
-        const double amp = 1.225 * sqrt(rho[i] * uc / ms);
+        const double amp = 1.225 * sqrt(rho[o] * uc / ms);
         const double m = exp(-cph * t.y), ang = cph * t.x;
         double sn, cs;
         sincos(ang, &sn, &cs);
@@ -118,22 +122,25 @@ static __global__ __launch_bounds__(256) void k_ic_assemble(const double *__rest
     }
 }
 
-// FP64 field -> storage precision with a power-of-two scale (mixed mode)
+// FP64 field -> storage precision with a power-of-two scale (mixed mode);
+// n2 cells per x-plane, planes sps (source) and dps (destination) apart
 template <typename T>
 __global__ __launch_bounds__(256) void k_store_scaled(const cx<double> *__restrict__ src, cx<T> *__restrict__ dst,
-                                                      long n3, int scale_exp) {
+                                                      long n3, long n2, long sps, long dps, int scale_exp) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
-        const cx<double> z = src[i];
-        dst[i] = mkc<T>((T)ldexp(z.x, -scale_exp), (T)ldexp(z.y, -scale_exp));
+        const long x = i / n2, r = i - x * n2;
+        const cx<double> z = src[x * sps + r];
+        dst[x * dps + r] = mkc<T>((T)ldexp(z.x, -scale_exp), (T)ldexp(z.y, -scale_exp));
     }
 }
 
 // storage precision -> FP64 field with the inverse power-of-two scale
 static __global__ __launch_bounds__(256) void k_load_scaled(const cx<float> *__restrict__ src, cx<double> *__restrict__ dst,
-                                                     long n3, int scale_exp) {
+                                                     long n3, long n2, long sps, long dps, int scale_exp) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long)gridDim.x * blockDim.x) {
-        const cx<float> z = src[i];
-        dst[i] = mkc<double>(ldexp((double)z.x, scale_exp), ldexp((double)z.y, scale_exp));
+        const long x = i / n2, r = i - x * n2;
+        const cx<float> z = src[x * sps + r];
+        dst[x * dps + r] = mkc<double>(ldexp((double)z.x, scale_exp), ldexp((double)z.y, scale_exp));
     }
 }
 

@@ -36,6 +36,7 @@ struct Ops {
         z.blk0 = 0;
         z.dtab = nullptr;
         z.rev = 0;
+        z.ps = c->ps;
         return z;
     }
     static SArgs sargs(ifdm_ctx *c, double dKE) {
@@ -94,7 +95,8 @@ struct Ops {
     // WIDE: 256-byte row tiles in FP64 (the x pass on psi; the half spectrum
     // keeps 128-byte tiles, whose last tile holds a single valid column)
     template <int AXIS, int MODE, bool WIDE = false>
-    static int spass(ifdm_ctx *c, void *arr, int rl, int kv, SArgs sa, int o0 = 0, int no = 0) {
+    // rl: row stride, ps: plane stride, kv: valid columns per row
+    static int spass(ifdm_ctx *c, void *arr, int rl, long ps, int kv, SArgs sa, int o0 = 0, int no = 0) {
         if (no == 0) no = N;
         sa.other0 = o0;
         int rc;
@@ -119,14 +121,14 @@ struct Ops {
             const int ntiles = ntz * no;
             const int nb = std::min(ntiles, occ * c->num_cu);
             hipLaunchKernelGGL((k_spass_pp<T, N, AXIS, MODE>), dim3(nb), dim3(TC * P), 0, c->stream,
-                               (cx<T> *)arr, rl, kv, (const cx<T> *)c->tw, sa, ntz, ntiles);
+                               (cx<T> *)arr, rl, ps, kv, (const cx<T> *)c->tw, sa, ntz, ntiles);
           }
         }
         if (!done) {
             constexpr int TC = s_cols<T, N, WIDE>();
             const int ntz = (kv + TC - 1) / TC;
             hipLaunchKernelGGL((k_spass<T, N, AXIS, MODE, WIDE>), dim3(ntz, no), dim3(TC * P), 0, c->stream,
-                               (cx<T> *)arr, rl, kv, (const cx<T> *)c->tw, sa);
+                               (cx<T> *)arr, rl, ps, kv, (const cx<T> *)c->tw, sa);
         }
         LAUNCHCHK(c);
         return prof_end(c);
@@ -165,22 +167,22 @@ struct Ops {
             case 1: {
                 // 128-byte tiles here (two work-groups per CU in FP64 overlap the
                 // kick's sincos with loads: measured faster than 256-byte tiles)
-                if (aphi == 0.0 && arho == 0.0) return spass<1, S_DRIFT1>(c, c->psi, N, N, sa);
+                if (aphi == 0.0 && arho == 0.0) return spass<1, S_DRIFT1>(c, c->psi, N, c->ps, N, sa);
                 SArgs sk = sa;
                 sk.aphi = aphi;
                 sk.arho = arho * c->psc2;
-                return spass<1, S_KICK_DRIFT1>(c, c->psi, N, N, sk);
+                return spass<1, S_KICK_DRIFT1>(c, c->psi, N, c->ps, N, sk);
             }
-            case 2: return spass<0, S_DRIFT1, true>(c, c->psi, N, N, sa);
+            case 2: return spass<0, S_DRIFT1, true>(c, c->psi, N, c->ps, N, sa);
             case 3: {
                 ZArgs za = zargs(c, 0, 0);
                 za.dtab = c->dtab;
                 c->rho_parts = c->nz_blocks;
                 return zpass<Z_DRIFT_SRC>(c, za, c->part_rho);
             }
-            case 4: return spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sr);
-            case 5: return spass<0, S_GREEN>(c, c->shat, c->hs, N / 2 + 1, sa);
-            case 6: return spass<1, S_INV>(c, c->shat, c->hs, N / 2 + 1, sa);
+            case 4: return spass<1, S_FWD>(c, c->shat, c->hs, (long)N * c->hs, N / 2 + 1, sr);
+            case 5: return spass<0, S_GREEN>(c, c->shat, c->hs, (long)N * c->hs, N / 2 + 1, sa);
+            case 6: return spass<1, S_INV>(c, c->shat, c->hs, (long)N * c->hs, N / 2 + 1, sa);
             case 7: return c2r(c, 0, 0, 1);
             case 8: {
                 int rc;
@@ -210,14 +212,15 @@ struct Ops {
 
     // reverse of one half-kick (aphi, arho) on (psi, adj_g) with the current phi
     static int adj_kick(ifdm_ctx *c, double aphi, double arho) {
-        const unsigned nb = (unsigned)std::min<long>((c->n3 + 255) / 256, 65536);
+        // (elementwise over the padded extent: pads hold zeros)
+        const unsigned nb = (unsigned)std::min<long>((c->np + 255) / 256, 65536);
         hipLaunchKernelGGL((k_adj_src<T>), dim3(nb), dim3(256), 0, c->stream, (const cx<T> *)c->psi,
-                           (const cx<T> *)c->adj_g, (T *)c->adj_s, c->n3);
+                           (const cx<T> *)c->adj_g, (T *)c->adj_s, c->np);
         LAUNCHCHK(c);
         int rc;
         if ((rc = poisson_real(c, c->adj_s, c->adj_phs))) return rc;
         hipLaunchKernelGGL((k_adj_kick<T>), dim3(nb), dim3(256), 0, c->stream, (cx<T> *)c->psi, (cx<T> *)c->adj_g,
-                           (const T *)c->phi, (const T *)c->adj_phs, (const T *)c->adj_s, c->n3, aphi,
+                           (const T *)c->phi, (const T *)c->adj_phs, (const T *)c->adj_s, c->np, aphi,
                            arho * c->psc2);
         LAUNCHCHK(c);
         return 0;
@@ -229,8 +232,8 @@ struct Ops {
         int rc;
         c->rho_parts = c->nz_blocks;
         if ((rc = zpass<Z_SRC>(c, zargs(c, 0, 0), c->part_rho))) return rc;
-        if ((rc = spass<1, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sa))) return rc;
-        if ((rc = spass<0, S_FWD>(c, c->shat, c->hs, N / 2 + 1, sa))) return rc;
+        if ((rc = spass<1, S_FWD>(c, c->shat, c->hs, (long)N * c->hs, N / 2 + 1, sa))) return rc;
+        if ((rc = spass<0, S_FWD>(c, c->shat, c->hs, (long)N * c->hs, N / 2 + 1, sa))) return rc;
         HIPCHK(c, hipMemsetAsync(dev_out, 0, sizeof(double) * 3 * nbins, c->stream));
         const long nh = (long)N * N * (N / 2 + 1);
         const unsigned nb = (unsigned)std::min<long>((nh + 255) / 256, 2048);
@@ -266,9 +269,9 @@ struct Ops {
         const ZArgs z0 = zargs(c, 0, 0);
         int rc;
         if ((rc = zpass<Z_FWD>(c, z0, nullptr))) return rc;
-        if ((rc = spass<1, S_FWD, true>(c, c->psi, N, N, sa))) return rc;
-        if ((rc = spass<0, S_GREEN, true>(c, c->psi, N, N, sa))) return rc;
-        if ((rc = spass<1, S_INV, true>(c, c->psi, N, N, sa))) return rc;
+        if ((rc = spass<1, S_FWD, true>(c, c->psi, N, c->ps, N, sa))) return rc;
+        if ((rc = spass<0, S_GREEN, true>(c, c->psi, N, c->ps, N, sa))) return rc;
+        if ((rc = spass<1, S_INV, true>(c, c->psi, N, c->ps, N, sa))) return rc;
         return zpass<Z_INV>(c, z0, nullptr);
     }
 
@@ -288,19 +291,19 @@ struct Ops {
         int rc;
         if (sign < 0) {
             if ((rc = zpass<Z_FWD>(c, zargs(c, 0, 0), nullptr))) return rc;
-            if ((rc = spass<1, S_FWD, true>(c, c->psi, N, N, sa))) return rc;
-            return spass<0, S_FWD, true>(c, c->psi, N, N, sa);
+            if ((rc = spass<1, S_FWD, true>(c, c->psi, N, c->ps, N, sa))) return rc;
+            return spass<0, S_FWD, true>(c, c->psi, N, c->ps, N, sa);
         }
         if ((rc = zpass<Z_INV>(c, zargs(c, 0, 0), nullptr))) return rc;
-        if ((rc = spass<1, S_INV, true>(c, c->psi, N, N, sa))) return rc;
-        return spass<0, S_INV, true>(c, c->psi, N, N, sa);
+        if ((rc = spass<1, S_INV, true>(c, c->psi, N, c->ps, N, sa))) return rc;
+        return spass<0, S_INV, true>(c, c->psi, N, c->ps, N, sa);
     }
 
     static int kick(ifdm_ctx *c, double aphi, double arho) {
-        long nb = (c->n3 + 255) / 256;
+        long nb = (c->np + 255) / 256;
         if (nb > 65536) nb = 65536;
         hipLaunchKernelGGL((k_kick<T>), dim3((unsigned)nb), dim3(256), 0, c->stream, (cx<T> *)c->psi,
-                           (const T *)c->phi, c->n3, aphi, arho * c->psc2);
+                           (const T *)c->phi, c->np, aphi, arho * c->psc2);
         LAUNCHCHK(c);
         return 0;
     }
@@ -439,6 +442,7 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
         return IFDM_E_ARG;
     }
     if (!(cfg->xmax > 0) || !(cfg->mfdm > 0)) { g_create_err = "xmax and mfdm must be positive"; return IFDM_E_ARG; }
+    if (cfg->plane_pad > 65536) { g_create_err = "plane_pad must be at most 65536"; return IFDM_E_ARG; }
     Dispatch d;
     if (!get_dispatch(cfg->n, cfg->precision, &d)) { g_create_err = "unsupported size"; return IFDM_E_ARG; }
 
@@ -448,6 +452,16 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     c->n3 = (long)cfg->n * cfg->n * cfg->n;
     c->hs = hs_of(cfg->n);
     c->mixed = cfg->precision == IFDM_MIXED;
+    // x-planes of psi and phi start 32 cells apart beyond their N*N cells: a
+    // power-of-two plane stride sends the rows of an x-line to the same HBM
+    // channels (measured at 512^3: the x pass B 1.09 -> 0.96 ms in FP64,
+    // 0.59 -> 0.44 ms mixed; 15.4 -> 10.5 ms FP64 at 1024^3).  Pads round up
+    // to 8 cells, so rows stay 8-cell aligned, as the phi lines that pass I
+    // stores as complex pairs need.
+    long pad = cfg->plane_pad > 0 ? cfg->plane_pad : (cfg->plane_pad < 0 ? 0 : 32);
+    pad = (pad + 7) / 8 * 8;
+    c->ps = (long)cfg->n * cfg->n + pad;
+    c->np = (long)cfg->n * c->ps;
     // persistent work-groups measured faster on the FP64 passes G and H only
     c->persistent = c->mixed ? 0 : 24;
     const int n = c->n;
@@ -468,8 +482,8 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     const int per = std::min(256 / (n / pts_of(c->mixed, n)), n * n);
     c->nz_blocks = (n * n + per - 1) / per;
     struct { void **p; size_t bytes; } allocs[] = {
-        {&c->psi, cs * (size_t)c->n3},
-        {&c->phi, rs * (size_t)c->n3},
+        {&c->psi, cs * (size_t)c->np},
+        {&c->phi, rs * (size_t)c->np},
         {&c->shat, cs * (size_t)n * n * c->hs},
         {&c->tw, cs * (size_t)n},
         {(void **)&c->k2, ds * (size_t)n},
@@ -493,8 +507,8 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     const void *twsrc = c->mixed ? (const void *)twf.data() : (const void *)tw.data();
     if ((e = hipMemcpy(c->tw, twsrc, cs * n, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(c->k2, k2.data(), ds * n, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemset(c->psi, 0, cs * (size_t)c->n3)) != hipSuccess ||
-        (e = hipMemset(c->phi, 0, rs * (size_t)c->n3)) != hipSuccess ||
+        (e = hipMemset(c->psi, 0, cs * (size_t)c->np)) != hipSuccess ||
+        (e = hipMemset(c->phi, 0, rs * (size_t)c->np)) != hipSuccess ||
         (e = hipMemset(c->shat, 0, cs * (size_t)n * n * c->hs)) != hipSuccess)
         return fail(IFDM_E_HIP, std::string("init copy: ") + hipGetErrorString(e));
     const double mt = cfg->mfdm / std::pow(c_light, 2);
@@ -522,7 +536,8 @@ int ifdm_set_psi(ifdm_ctx *c, const double *psi) {
     HIPCHK(c, hipSetDevice(c->cfg.device));
     c->pending = false;
     if (!c->mixed) {
-        HIPCHK(c, hipMemcpyAsync(c->psi, psi, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, plane_copy(c, c->psi, const_cast<double *>(psi), 2 * sizeof(double), c->n, true,
+                             hipMemcpyHostToDevice));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return IFDM_OK;
     }
@@ -533,7 +548,8 @@ int ifdm_set_psi(ifdm_ctx *c, const double *psi) {
         norm += psi[i] * psi[i];
     }
     c->rho_mean = c->ms * norm / (double)c->n3;
-    HIPCHK(c, hipMemcpy(c->psi, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, plane_copy(c, c->psi, tmp.data(), 2 * sizeof(float), c->n, true, hipMemcpyHostToDevice));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return IFDM_OK;
 }
 
@@ -543,11 +559,13 @@ int ifdm_get_psi(ifdm_ctx *c, double *psi) {
     if (int rc0 = ifdm_internal_flush(c)) return rc0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (!c->mixed) {
-        HIPCHK(c, hipMemcpy(psi, c->psi, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToHost));
+        HIPCHK(c, plane_copy(c, c->psi, psi, 2 * sizeof(double), c->n, false, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         return IFDM_OK;
     }
     std::vector<float> tmp(2 * (size_t)c->n3);
-    HIPCHK(c, hipMemcpy(tmp.data(), c->psi, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(c, plane_copy(c, c->psi, tmp.data(), 2 * sizeof(float), c->n, false, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     for (size_t i = 0; i < tmp.size(); ++i) psi[i] = std::ldexp((double)tmp[i], c->psi_scale);
     return IFDM_OK;
 }
@@ -557,11 +575,13 @@ int ifdm_get_phi(ifdm_ctx *c, double *phi) {
     HIPCHK(c, hipSetDevice(c->cfg.device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (!c->mixed) {
-        HIPCHK(c, hipMemcpy(phi, c->phi, sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToHost));
+        HIPCHK(c, plane_copy(c, c->phi, phi, sizeof(double), c->n, false, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         return IFDM_OK;
     }
     std::vector<float> tmp((size_t)c->n3);
-    HIPCHK(c, hipMemcpy(tmp.data(), c->phi, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(c, plane_copy(c, c->phi, tmp.data(), sizeof(float), c->n, false, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     for (size_t i = 0; i < tmp.size(); ++i) phi[i] = tmp[i];
     return IFDM_OK;
 }
@@ -572,12 +592,14 @@ int ifdm_set_phi(ifdm_ctx *c, const double *phi) {
     if (int rc0 = ifdm_internal_flush(c)) return rc0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (!c->mixed) {
-        HIPCHK(c, hipMemcpy(c->phi, phi, sizeof(double) * (size_t)c->n3, hipMemcpyHostToDevice));
+        HIPCHK(c, plane_copy(c, c->phi, const_cast<double *>(phi), sizeof(double), c->n, true, hipMemcpyHostToDevice));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         return IFDM_OK;
     }
     std::vector<float> tmp((size_t)c->n3);
     for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)phi[i];
-    HIPCHK(c, hipMemcpy(c->phi, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, plane_copy(c, c->phi, tmp.data(), sizeof(float), c->n, true, hipMemcpyHostToDevice));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return IFDM_OK;
 }
 
@@ -725,6 +747,8 @@ int ifdm_internal_theta_solve(ifdm_ctx *c, void *buf64, double coef) {
     sh.cfg.precision = IFDM_FP64;
     sh.n = c->n;
     sh.n3 = c->n3;
+    sh.ps = c->ps;
+    sh.np = c->np;
     sh.hs = c->hs;
     sh.stream = c->stream;
     sh.psi = buf64;
@@ -784,7 +808,7 @@ int ifdm_run_checkpointed(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, int K) 
     HIPCHK(c, hipSetDevice(c->cfg.device));
     int rc;
     if ((rc = ifdm_internal_flush(c))) return rc;
-    const size_t bytes = (c->mixed ? 2 * sizeof(float) : 2 * sizeof(double)) * (size_t)c->n3;
+    const size_t bytes = (c->mixed ? 2 * sizeof(float) : 2 * sizeof(double)) * (size_t)c->np;
     if (c->ck) {
         hipFree(c->ck);
         c->ck = nullptr;
@@ -817,7 +841,7 @@ int ifdm_run_checkpointed(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, int K) 
 static int adjoint_sweep(ifdm_ctx *c, const ifdm_dt *sched, int nsteps) {
     Dispatch d;
     get_dispatch(c->n, c->cfg.precision, &d);
-    const size_t bytes = (c->mixed ? 2 * sizeof(float) : 2 * sizeof(double)) * (size_t)c->n3;
+    const size_t bytes = (c->mixed ? 2 * sizeof(float) : 2 * sizeof(double)) * (size_t)c->np;
     int rc;
     if ((rc = d.poisson(c))) return rc;                      // phi of the final state
     for (int n = nsteps - 1; n >= 0; --n) {
@@ -846,13 +870,17 @@ static int adjoint_prepare(ifdm_ctx *c, int nsteps) {
         return IFDM_E_STATE;
     }
     const size_t es = c->mixed ? sizeof(float) : sizeof(double);
-    const size_t cs = 2 * es * (size_t)c->n3, rs = es * (size_t)c->n3;
+    const size_t cs = 2 * es * (size_t)c->np, rs = es * (size_t)c->np;
     if (!c->adj_g) {
         if (hipMalloc(&c->adj_g, cs) != hipSuccess || hipMalloc(&c->adj_s, rs) != hipSuccess ||
             hipMalloc(&c->adj_phs, rs) != hipSuccess) {
             c->err = "hipMalloc failed for the adjoint fields";
             return IFDM_E_NOMEM;
         }
+        // the plane pads stay zero through the elementwise adjoint kernels
+        HIPCHK(c, hipMemset(c->adj_g, 0, cs));
+        HIPCHK(c, hipMemset(c->adj_s, 0, rs));
+        HIPCHK(c, hipMemset(c->adj_phs, 0, rs));
     }
     return ifdm_internal_flush(c);
 }
@@ -864,9 +892,10 @@ int ifdm_adjoint(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, double *g) {
     if ((rc = adjoint_prepare(c, nsteps))) return rc;
     const size_t n2 = 2 * (size_t)c->n3;
     if (!c->mixed) {
-        HIPCHK(c, hipMemcpy(c->adj_g, g, n2 * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c, plane_copy(c, c->adj_g, g, 2 * sizeof(double), c->n, true, hipMemcpyHostToDevice));
         if ((rc = adjoint_sweep(c, sched, nsteps))) return rc;
-        HIPCHK(c, hipMemcpy(g, c->adj_g, n2 * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, plane_copy(c, c->adj_g, g, 2 * sizeof(double), c->n, false, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         return IFDM_OK;
     }
     // mixed: the sweep is linear in g, so g is held as 2^-sg g in FP32 with
@@ -876,9 +905,10 @@ int ifdm_adjoint(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, double *g) {
     const int sg = gmax > 0 ? std::ilogb(gmax) + 1 : 0;
     std::vector<float> tmp(n2);
     for (size_t i = 0; i < n2; ++i) tmp[i] = (float)std::ldexp(g[i], -sg);
-    HIPCHK(c, hipMemcpy(c->adj_g, tmp.data(), n2 * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, plane_copy(c, c->adj_g, tmp.data(), 2 * sizeof(float), c->n, true, hipMemcpyHostToDevice));
     if ((rc = adjoint_sweep(c, sched, nsteps))) return rc;
-    HIPCHK(c, hipMemcpy(tmp.data(), c->adj_g, n2 * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(c, plane_copy(c, c->adj_g, tmp.data(), 2 * sizeof(float), c->n, false, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     // back to physical units: g scale 2^sg, and the psi scale of the kicked
     // field enters g = e^{i theta} g' + 2 psi A s linearly through psi and s
     for (size_t i = 0; i < n2; ++i) g[i] = std::ldexp((double)tmp[i], sg);
@@ -891,10 +921,9 @@ int ifdm_adjoint_device(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, void *g_d
     HIPCHK(c, hipSetDevice(c->cfg.device));
     int rc;
     if ((rc = adjoint_prepare(c, nsteps))) return rc;
-    const size_t cs = 2 * sizeof(double) * (size_t)c->n3;
-    HIPCHK(c, hipMemcpyAsync(c->adj_g, g_dev, cs, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, plane_copy(c, c->adj_g, g_dev, 2 * sizeof(double), c->n, true, hipMemcpyDeviceToDevice));
     if ((rc = adjoint_sweep(c, sched, nsteps))) return rc;
-    HIPCHK(c, hipMemcpyAsync(g_dev, c->adj_g, cs, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, plane_copy(c, c->adj_g, g_dev, 2 * sizeof(double), c->n, false, hipMemcpyDeviceToDevice));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return IFDM_OK;
 }
@@ -903,13 +932,14 @@ int ifdm_set_psi_device(ifdm_ctx *c, const void *psi_dev) {
     if (!c || !psi_dev) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     c->pending = false;
+    const long n2 = (long)c->n * c->n;
     if (!c->mixed) {
-        HIPCHK(c, hipMemcpyAsync(c->psi, psi_dev, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToDevice,
-                                 c->stream));
+        HIPCHK(c, plane_copy(c, c->psi, const_cast<void *>(psi_dev), 2 * sizeof(double), c->n, true,
+                             hipMemcpyDeviceToDevice));
     } else {
         const unsigned nb = (unsigned)std::min<long>((c->n3 + 255) / 256, 65536);
         hipLaunchKernelGGL((k_store_scaled<float>), dim3(nb), dim3(256), 0, c->stream, (const cx<double> *)psi_dev,
-                           (cx<float> *)c->psi, c->n3, c->psi_scale);
+                           (cx<float> *)c->psi, c->n3, n2, n2, c->ps, c->psi_scale);
         LAUNCHCHK(c);
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -920,13 +950,13 @@ int ifdm_get_psi_device(ifdm_ctx *c, void *psi_dev) {
     if (!c || !psi_dev) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     if (int rc0 = ifdm_internal_flush(c)) return rc0;
+    const long n2 = (long)c->n * c->n;
     if (!c->mixed) {
-        HIPCHK(c, hipMemcpyAsync(psi_dev, c->psi, 2 * sizeof(double) * (size_t)c->n3, hipMemcpyDeviceToDevice,
-                                 c->stream));
+        HIPCHK(c, plane_copy(c, c->psi, psi_dev, 2 * sizeof(double), c->n, false, hipMemcpyDeviceToDevice));
     } else {
         const unsigned nb = (unsigned)std::min<long>((c->n3 + 255) / 256, 65536);
         hipLaunchKernelGGL(k_load_scaled, dim3(nb), dim3(256), 0, c->stream, (const cx<float> *)c->psi,
-                           (cx<double> *)psi_dev, c->n3, c->psi_scale);
+                           (cx<double> *)psi_dev, c->n3, n2, c->ps, n2, c->psi_scale);
         LAUNCHCHK(c);
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -153,29 +153,34 @@ int ifdm_ic_finish(ifdm_ctx *c, double a, double *theta_out, float *vx_out, long
         HIPCHK(c, hipStreamSynchronize(c->stream));
         for (long i = 0; i < n3; ++i) vx_out[i] = (float)tmp[i];
     }
-    // theta work buffer: psi itself in FP64 mode
+    // theta work buffer (psi's plane stride): psi itself in FP64 mode
     void *buf = c->psi;
-    if (c->mixed) HIPCHK(c, hipMalloc(&buf, 2 * sizeof(double) * (size_t)n3));
+    if (c->mixed) {
+        HIPCHK(c, hipMalloc(&buf, 2 * sizeof(double) * (size_t)c->np));
+        HIPCHK(c, hipMemsetAsync(buf, 0, 2 * sizeof(double) * (size_t)c->np, c->stream));
+    }
     const double dx = 2.0 * c->cfg.xmax / c->n;                          // par.dx, src/struct3D.h:52
     const double unit = 1e3 / m2Mpc * s2Myr;                              // src/gadget_IC.h:171
     hipLaunchKernelGGL(k_ic_divergence, dim3(grid_for(n3)), dim3(256), 0, c->stream, c->ic_vx, c->ic_vy, c->ic_vz,
-                       (cx<double> *)buf, c->n, unit / dx);
+                       (cx<double> *)buf, c->n, c->ps, unit / dx);
     int rc = hipGetLastError() == hipSuccess ? IFDM_OK : IFDM_E_HIP;
     if (!rc) rc = ifdm_internal_theta_solve(c, buf, -std::pow(a, 3. / 2));   // src/gadget_IC.h:233
     if (!rc && theta_out) {
-        if (hipMemcpy(theta_out, buf, 2 * sizeof(double) * (size_t)n3, hipMemcpyDeviceToHost) != hipSuccess)
+        if (plane_copy(c, buf, theta_out, 2 * sizeof(double), c->n, false, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
             rc = IFDM_E_HIP;
     }
     if (!rc) {
         const double uc = std::pow(h_0, 2) * std::pow(1e3, 3);            // src/gadget_IC.h:659
         const double cph = c->cfg.mfdm / std::pow(c_light, 2) / hbar;
         hipLaunchKernelGGL(k_ic_assemble, dim3(grid_for(n3)), dim3(256), 0, c->stream, c->ic_rho, (cx<double> *)buf,
-                           n3, uc, c->ms, cph);
+                           c->n, c->ps, uc, c->ms, cph);
         if (hipGetLastError() != hipSuccess) rc = IFDM_E_HIP;
     }
     if (!rc && c->mixed) {
         hipLaunchKernelGGL((k_store_scaled<float>), dim3(grid_for(n3)), dim3(256), 0, c->stream,
-                           (const cx<double> *)buf, (cx<float> *)c->psi, n3, c->psi_scale);
+                           (const cx<double> *)buf, (cx<float> *)c->psi, n3, (long)c->n * c->n, c->ps, c->ps,
+                           c->psi_scale);
         if (hipGetLastError() != hipSuccess) rc = IFDM_E_HIP;
     }
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = IFDM_E_HIP;
@@ -296,9 +301,13 @@ int ifdm_write_snapshot(ifdm_ctx *c, const char *path, double a, double mfdm_ev,
     for (size_t p0 = 0; p0 < (size_t)n; p0 += planes_per) {
         const size_t np = std::min(planes_per, (size_t)n - p0), cnt = 2 * plane * np;
         if (!c->mixed) {
-            HIPCHK(c, hipMemcpy(buf.data(), (double *)c->psi + 2 * plane * p0, cnt * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(c, plane_copy(c, (double *)c->psi + 2 * c->ps * p0, buf.data(), 2 * sizeof(double), (int)np, false,
+                                 hipMemcpyDeviceToHost));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
         } else {
-            HIPCHK(c, hipMemcpy(fbuf.data(), (float *)c->psi + 2 * plane * p0, cnt * sizeof(float), hipMemcpyDeviceToHost));
+            HIPCHK(c, plane_copy(c, (float *)c->psi + 2 * c->ps * p0, fbuf.data(), 2 * sizeof(float), (int)np, false,
+                                 hipMemcpyDeviceToHost));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
             for (size_t i = 0; i < cnt; ++i) buf[i] = std::ldexp((double)fbuf[i], c->psi_scale);
         }
         if (std::fwrite(buf.data(), sizeof(double), cnt, f.f) != cnt) {
@@ -334,13 +343,17 @@ int ifdm_read_snapshot(ifdm_ctx *c, const char *path, double *a_out, double *mfd
             return IFDM_E_FORMAT;
         }
         if (!c->mixed) {
-            HIPCHK(c, hipMemcpy((double *)c->psi + 2 * plane * p0, buf.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(c, plane_copy(c, (double *)c->psi + 2 * c->ps * p0, buf.data(), 2 * sizeof(double), (int)np, true,
+                                 hipMemcpyHostToDevice));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
         } else {
             for (size_t i = 0; i < cnt; ++i) {
                 fbuf[i] = (float)std::ldexp(buf[i], -c->psi_scale);
                 norm += buf[i] * buf[i];
             }
-            HIPCHK(c, hipMemcpy((float *)c->psi + 2 * plane * p0, fbuf.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(c, plane_copy(c, (float *)c->psi + 2 * c->ps * p0, fbuf.data(), 2 * sizeof(float), (int)np, true,
+                                 hipMemcpyHostToDevice));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
         }
     }
     c->pending = false;

@@ -40,13 +40,14 @@ struct ZArgs {
                       // mean m_s <|psi|^2> in mixed mode)
     int ref_range;    // max|phi| over the reference's one-rank scan range
     int blk0;         // first block of this launch
-    void *dtab;       // drift table E[k_z] (the z drift of pass A)
+    void *dtab;       // drift table E[k_z] (the z drift of pass C)
     int rev;          // blocks walk the lines in descending order
+    long ps;          // plane stride (elements) of psi and phi: N*N + padding
 };
 
 struct SArgs {
     int other0;        // first line index of the fixed axis in this launch
-    const void *dtab;  // drift table E[i] (pass C)
+    const void *dtab;  // drift table E[i] (passes A, B)
     const double *k2;  // k_i^2, i < N (src/struct3D.h:76-85 squared)
     double ke_coef;    // 0.5 hbar / (m / c^2)   (src/struct3D.h:121)
     double dKE;        // time integral of the drift
@@ -78,6 +79,14 @@ enum ZMode { Z_FWD = 0, Z_INV = 1, Z_SRC = 2, Z_C2R = 3, Z_RSRC = 4, Z_DRIFT = 5
 // S_KICK_DRIFT1: the kick exp(-i (aphi phi + arho |psi|^2)) first;
 // S_GREEN: forward, Green's function, inverse
 enum SMode { S_FWD = 0, S_INV = 1, S_GREEN = 2, S_DRIFT1 = 3, S_KICK_DRIFT1 = 4 };
+
+// Offset of z-line l = x N + y in a field of plane stride ps (elements):
+// psi and phi keep each x-plane's N*N cells contiguous and start planes ps
+// elements apart.  A plane stride of N*N (a power of two) maps the rows of
+// an x-line to the same HBM channels; a small pad spreads them.
+template <int N> __device__ __forceinline__ long zline_off(long l, long ps) {
+    return (l / N) * ps + (l % N) * (long)N;
+}
 
 template <typename T, int N> constexpr int z_lines() {
     constexpr int P = N / SchedT<T, N>::E;
@@ -167,7 +176,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         }
         fft_line<T, N, +1, wave_line<T, N>()>(v, L, t, tb);
         double mx = 0.0;
-        T *out = phi + line * N;
+        T *out = phi + zline_off<N>(line, za.ps);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int p = t + e * P;
@@ -183,7 +192,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         return;
     } else if constexpr (MODE == Z_RSRC) {
         // Poisson source from a real field r (read through `phi`): ms * r
-        const T *rrow = phi + line * N;
+        const T *rrow = phi + zline_off<N>(line, za.ps);
         cx<T> s[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) s[e] = mkc<T>(valid ? (T)(za.ms * (double)rrow[t + e * P]) : (T)0, (T)0);
@@ -198,7 +207,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         }
         return;
     } else {
-        cx<T> *row = psi + line * N;
+        cx<T> *row = psi + zline_off<N>(line, za.ps);
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = valid ? row[t + e * P] : mkc<T>(0, 0);
 
@@ -304,7 +313,7 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
     }
     fft_line<T, H, +1, wave_line<T, H>()>(v, L, t, tb);
     double mx = 0.0;
-    cx<T> *out = reinterpret_cast<cx<T> *>(phi + line * N);
+    cx<T> *out = reinterpret_cast<cx<T> *>(phi + zline_off<N>(line, za.ps));
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int m = t + e * P;
@@ -364,9 +373,9 @@ __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const T
 }
 
 template <int N, int AXIS>
-__device__ __forceinline__ void spass_addr(int other, int k, int rl, long &base, long &stride) {
-    if constexpr (AXIS == 1) { base = (long)other * N * rl + k; stride = rl; }
-    else { base = (long)other * rl + k; stride = (long)N * rl; }
+__device__ __forceinline__ void spass_addr(int other, int k, int rl, long ps, long &base, long &stride) {
+    if constexpr (AXIS == 1) { base = (long)other * ps + k; stride = rl; }
+    else { base = (long)other * rl + k; stride = ps; }
 }
 
 // Element access within one tile: a wave-uniform tile base (scalar
@@ -395,7 +404,7 @@ template <typename T, int N> struct TileIO {
 // one tile per work-group: grid (z-column tiles, lines of the fixed axis)
 template <typename T, int N, int AXIS, int MODE, bool WIDE>
 __global__ __launch_bounds__((s_cols<T, N, WIDE>() * (N / SchedT<T, N>::E)))
-void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw, SArgs sa) {
+void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, SArgs sa) {
     constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE>(), LS = lds_line<N>();
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
@@ -403,7 +412,7 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
     const int other = sa.other0 + block_order(sa.rev, blockIdx.y, gridDim.y);
     const bool valid = k < kv;
     long base0, stride;
-    spass_addr<N, AXIS>(other, k0, rl, base0, stride);
+    spass_addr<N, AXIS>(other, k0, rl, ps, base0, stride);
     const TileIO<T, N> io(a, base0);
     cx<T> *L = lds + c * LS;
     TwBase<T, N> tb;
@@ -434,7 +443,7 @@ void k_spass(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw
 // transform is preceded by a barrier, so consecutive tiles never race on LDS.
 template <typename T, int N, int AXIS, int MODE>
 __global__ __launch_bounds__((s_cols<T, N>() * (N / SchedT<T, N>::E)))
-void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__ tw, SArgs sa, int ntz,
+void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, SArgs sa, int ntz,
                 int ntiles) {
     constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N>(), LS = lds_line<N>();
     constexpr int NT = TC * P;
@@ -460,7 +469,7 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__
     }
     const cx<T> *tab = LT ? (const cx<T> *)tabL : (const cx<T> *)sa.dtab;
     const double *k2t = LT ? (const double *)k2L : sa.k2;
-    const long stride = AXIS == 1 ? (long)rl : (long)N * rl;
+    const long stride = AXIS == 1 ? (long)rl : ps;
     // With at least TC columns per line every column of every tile lies
     // inside its row (rl >= the tile span; columns k >= kv are the half
     // spectrum's zero padding), so loads and stores need no condition and
@@ -473,7 +482,7 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__
     {
         const int other = sa.other0 + tile / ntz, k0 = (tile % ntz) * TC;
         long b0, st;
-        spass_addr<N, AXIS>(other, k0, rl, b0, st);
+        spass_addr<N, AXIS>(other, k0, rl, ps, b0, st);
         const TileIO<T, N> io(a, b0);
         const bool valid = k0 + c < kv;
 #pragma unroll
@@ -492,7 +501,7 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__
             const int lt = more ? nt : tile;
             const int o2 = sa.other0 + lt / ntz, k2 = (lt % ntz) * TC;
             long b2, st;
-            spass_addr<N, AXIS>(o2, k2, rl, b2, st);
+            spass_addr<N, AXIS>(o2, k2, rl, ps, b2, st);
             const TileIO<T, N> io(a, b2);
             const bool valid2 = k2 + c < kv;
 #pragma unroll
@@ -504,7 +513,7 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, int kv, const cx<T> *__restrict__
         spass_compute<T, N, MODE, false>(v, L, t, tb, sa, other, k, valid, tab, k2t);
         {
             long b0, st;
-            spass_addr<N, AXIS>(other, k0, rl, b0, st);
+            spass_addr<N, AXIS>(other, k0, rl, ps, b0, st);
             const TileIO<T, N> io(a, b0);
             if (FULL || valid) {
 #pragma unroll

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define IFDM_ABI_VERSION 1
+#define IFDM_ABI_VERSION 2
 
 enum {
     IFDM_OK = 0,
@@ -54,6 +54,11 @@ typedef struct {
                               reference's padded phi array (its one-rank
                               std::max_element range); 0: all cells */
     int device;            /* HIP device ordinal */
+    int plane_pad;         /* device layout of psi and phi: cells of padding
+                              after each x-plane of N*N cells, rounded up to
+                              a multiple of 8 (0: the built-in 32 cells;
+                              negative: none; at most 65536).  Host arrays
+                              stay dense. */
 } ifdm_config;
 
 /* Reductions after a Poisson solve / step: the two scans of

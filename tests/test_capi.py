@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_constants():
     import ifdm_code_amd as F
     from oracle import oracle as O
-    assert F.lib().ifdm_abi_version() == 1
+    assert F.lib().ifdm_abi_version() == 2
     for name, val in [("hbar", O.hbar), ("c", O.c_light), ("bigG", O.bigG), ("H_0", O.H_0),
                       ("h_0", O.h_0), ("Omega_m0", O.Omega_m0), ("rhoc", O.rhoc_const),
                       ("cm2Mpc", O.cm2Mpc), ("kg2eV", O.kg2eV), ("kg2solar", O.kg2solar)]:

@@ -279,6 +279,44 @@ def test_persistent_schedule_is_identical(n, prec):
     assert rel_l2(outs[1][1], outs[0][1]) < (1e-12 if prec == 0 else 1e-5)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_plane_padding_is_layout_only(prec, tmp_path):
+    """psi and phi live on the device as x-planes of N*N cells with a pad
+    after each plane (config plane_pad).  The pad moves planes, not the
+    arithmetic: the adaptive schedule, psi, phi, the adjoint gradient and a
+    snapshot read back into the other layout agree bit for bit with the
+    unpadded layout, for the built-in pad, an odd pad and a large one."""
+    import dataclasses
+    F, _O = _mods()
+    n = 32
+    cfg0 = F.Config.from_cli(2.5e-24, G_IN, 5.0, n, precision=prec)
+    psi0 = make_psi(n, 5, cfg0.mfdm_solar)
+    rng = np.random.default_rng(11)
+    g_t = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
+    res = {}
+    for pad in (-1, 0, 5, 1000):
+        _say(f"[plane pad] precision={prec} pad={pad}")
+        snap = str(tmp_path / f"snap_{pad}.dat")
+        with F.Solver(dataclasses.replace(cfg0, plane_pad=pad)) as s:
+            s.set_psi(psi0)
+            a, sched = s.simulate_native(1.0 / 51, 4)
+            psi, phi = s.get_psi(), s.get_phi()
+            s.write_snapshot(snap, a, 2.5e-24, 0.0)
+            s.set_psi(psi0)
+            s.poisson()
+            s.run_checkpointed(sched, 2)
+            grad = s.adjoint(sched, g_t)
+        with F.Solver(dataclasses.replace(cfg0, plane_pad=0 if pad < 0 else -1)) as s2:
+            s2.read_snapshot(snap)
+            back = s2.get_psi()
+        res[pad] = (np.array(sched), psi, phi, grad, back)
+    ref = res[-1]
+    assert np.array_equal(ref[4], ref[1])
+    for pad, r in res.items():
+        for what, x, y in zip(("schedule", "psi", "phi", "gradient", "snapshot"), r, ref):
+            assert np.array_equal(x, y), (pad, what)
+
+
 def test_golden_oracle_trajectory():
     """Three adaptive steps from the seeded 16^3 field of the committed
     oracle fixture (tests/golden/kdk_16_oracle.npz): same schedule and state
