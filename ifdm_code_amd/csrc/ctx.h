@@ -15,11 +15,13 @@ struct ifdm_ctx {
     // psi, phi and the adjoint fields hold x-plane after x-plane, each plane
     // N*N contiguous cells, planes ps cells apart (ps = N*N + pad); np = N*ps
     long ps = 0, np = 0;
+    // half spectrum: rows of hs cells, N rows per x-plane, planes sps apart
     int hs = 0;
+    long sps = 0;
     hipStream_t stream = nullptr;
     void *psi = nullptr;       // N^3 complex (plane stride ps)
     void *phi = nullptr;       // N^3 real (plane stride ps)
-    void *shat = nullptr;      // N*N*HS complex
+    void *shat = nullptr;      // N planes of N*HS complex (plane stride sps)
     void *tw = nullptr;        // N complex twiddles
     double *k2 = nullptr;      // N doubles
     double *part_rho = nullptr;

@@ -37,6 +37,7 @@ struct Ops {
         z.dtab = nullptr;
         z.rev = 0;
         z.ps = c->ps;
+        z.sps = c->sps;
         return z;
     }
     static SArgs sargs(ifdm_ctx *c, double dKE) {
@@ -180,9 +181,9 @@ struct Ops {
                 c->rho_parts = c->nz_blocks;
                 return zpass<Z_DRIFT_SRC>(c, za, c->part_rho);
             }
-            case 4: return spass<1, S_FWD>(c, c->shat, c->hs, (long)N * c->hs, N / 2 + 1, sr);
-            case 5: return spass<0, S_GREEN>(c, c->shat, c->hs, (long)N * c->hs, N / 2 + 1, sa);
-            case 6: return spass<1, S_INV>(c, c->shat, c->hs, (long)N * c->hs, N / 2 + 1, sa);
+            case 4: return spass<1, S_FWD>(c, c->shat, c->hs, c->sps, N / 2 + 1, sr);
+            case 5: return spass<0, S_GREEN>(c, c->shat, c->hs, c->sps, N / 2 + 1, sa);
+            case 6: return spass<1, S_INV>(c, c->shat, c->hs, c->sps, N / 2 + 1, sa);
             case 7: return c2r(c, 0, 0, 1);
             case 8: {
                 int rc;
@@ -232,13 +233,13 @@ struct Ops {
         int rc;
         c->rho_parts = c->nz_blocks;
         if ((rc = zpass<Z_SRC>(c, zargs(c, 0, 0), c->part_rho))) return rc;
-        if ((rc = spass<1, S_FWD>(c, c->shat, c->hs, (long)N * c->hs, N / 2 + 1, sa))) return rc;
-        if ((rc = spass<0, S_FWD>(c, c->shat, c->hs, (long)N * c->hs, N / 2 + 1, sa))) return rc;
+        if ((rc = spass<1, S_FWD>(c, c->shat, c->hs, c->sps, N / 2 + 1, sa))) return rc;
+        if ((rc = spass<0, S_FWD>(c, c->shat, c->hs, c->sps, N / 2 + 1, sa))) return rc;
         HIPCHK(c, hipMemsetAsync(dev_out, 0, sizeof(double) * 3 * nbins, c->stream));
         const long nh = (long)N * N * (N / 2 + 1);
         const unsigned nb = (unsigned)std::min<long>((nh + 255) / 256, 2048);
         hipLaunchKernelGGL((k_pk_bin<T, N>), dim3(nb), dim3(256), sizeof(double) * 3 * nbins, c->stream,
-                           (const cx<T> *)c->shat, c->hs, nbins, dev_out);
+                           (const cx<T> *)c->shat, c->hs, c->sps, nbins, dev_out);
         LAUNCHCHK(c);
         c->cur_pass = 8;
         return pass(c, 8, 0, 0, 0);   // reductions (the norm gives the mean density)
@@ -462,6 +463,9 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     pad = (pad + 7) / 8 * 8;
     c->ps = (long)cfg->n * cfg->n + pad;
     c->np = (long)cfg->n * c->ps;
+    // (the half spectrum's plane stride N*HS is no power of two: padding it
+    // measured neutral)
+    c->sps = (long)cfg->n * c->hs;
     // persistent work-groups measured faster on the FP64 passes G and H only
     c->persistent = c->mixed ? 0 : 24;
     const int n = c->n;
@@ -484,7 +488,7 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     struct { void **p; size_t bytes; } allocs[] = {
         {&c->psi, cs * (size_t)c->np},
         {&c->phi, rs * (size_t)c->np},
-        {&c->shat, cs * (size_t)n * n * c->hs},
+        {&c->shat, cs * (size_t)n * c->sps},
         {&c->tw, cs * (size_t)n},
         {(void **)&c->k2, ds * (size_t)n},
         {(void **)&c->part_rho, 2 * ds * (size_t)c->nz_blocks},
@@ -509,7 +513,7 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
         (e = hipMemcpy(c->k2, k2.data(), ds * n, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemset(c->psi, 0, cs * (size_t)c->np)) != hipSuccess ||
         (e = hipMemset(c->phi, 0, rs * (size_t)c->np)) != hipSuccess ||
-        (e = hipMemset(c->shat, 0, cs * (size_t)n * n * c->hs)) != hipSuccess)
+        (e = hipMemset(c->shat, 0, cs * (size_t)n * c->sps)) != hipSuccess)
         return fail(IFDM_E_HIP, std::string("init copy: ") + hipGetErrorString(e));
     const double mt = cfg->mfdm / std::pow(c_light, 2);
     c->m_over_hbar = mt / hbar;
@@ -750,6 +754,7 @@ int ifdm_internal_theta_solve(ifdm_ctx *c, void *buf64, double coef) {
     sh.ps = c->ps;
     sh.np = c->np;
     sh.hs = c->hs;
+    sh.sps = c->sps;
     sh.stream = c->stream;
     sh.psi = buf64;
     sh.k2 = c->k2;

@@ -1,22 +1,23 @@
 // kdk_passes.h -- the fused pencil passes of one kick-drift-kick step.
 //
-// Fields in HBM (C order, z fastest, as the reference's opr.wfc):
-//   psi   N^3 complex                 (in-place spectral work)
-//   phi   N^3 real, unpadded
-//   shat  N * N * HS complex: half spectrum of the Poisson source, z extent
-//         N/2+1 valid entries padded to HS (a multiple of 8) so that each
-//         row starts on a 128-byte boundary.
+// Fields in HBM (x, y, z with z fastest, as the reference's opr.wfc):
+//   psi   N^3 complex, x-planes of N*N cells ps cells apart (ps = N*N + pad)
+//   phi   N^3 real, the same plane stride
+//   shat  half spectrum of the Poisson source: rows of HS cells (N/2+1
+//         valid, HS a multiple of 16, so rows start on 128-byte
+//         boundaries), N rows per x-plane, planes sps cells apart
 //
-// One step is nine passes (A..I), each one read and one write of a field:
-//   A  z:  kick (phase from phi and |psi|^2), forward FFT            psi
-//   B  y:  forward FFT                                                psi
-//   C  x:  forward FFT, drift exp(-i KE dKE)/N^3, inverse FFT         psi
-//   D  y:  inverse FFT                                                psi
-//   E  z:  inverse FFT -> psi; m_s|psi|^2 - rho_c -> real FFT -> shat;
-//          block max |psi|^2 and sum |psi|^2
-//   F  y:  forward FFT                                                shat
-//   G  x:  forward FFT, Green -4 pi G / k^2 / N^3 (k = 0 -> 0), inverse shat
-//   H  y:  inverse FFT                                                shat
+// One step is a drift table and eight passes, each one read and one write
+// of a field (the drift factor separates by axis, so each axis makes its
+// own forward - factor - inverse round trip):
+//   T      drift table E[i] = exp(-i c k_i^2 dKE)
+//   A  y:  kick (phase from phi and |psi|^2), forward, E[k_y]/N, inverse  psi
+//   B  x:  forward, E[k_x]/N, inverse                                      psi
+//   C  z:  forward, E[k_z]/N, inverse -> psi; m_s|psi|^2 - rho_c -> real
+//          FFT -> shat; block max |psi|^2 and sum |psi|^2
+//   F  y:  forward FFT                                                     shat
+//   G  x:  forward FFT, Green -4 pi G / k^2 / N^3 (k = 0 -> 0), inverse    shat
+//   H  y:  inverse FFT                                                     shat
 //   I  z:  complex-to-real inverse FFT -> phi; block max |phi|
 // The reference's order is src/CosmoBEC_MPI_stan.cpp:130-152 with
 // poisson() at :80-119; its drift multiplies by fftnorm at :141 and its
@@ -43,6 +44,7 @@ struct ZArgs {
     void *dtab;       // drift table E[k_z] (the z drift of pass C)
     int rev;          // blocks walk the lines in descending order
     long ps;          // plane stride (elements) of psi and phi: N*N + padding
+    long sps;         // plane stride (elements) of shat: N*HS + padding
 };
 
 struct SArgs {
@@ -86,6 +88,11 @@ enum SMode { S_FWD = 0, S_INV = 1, S_GREEN = 2, S_DRIFT1 = 3, S_KICK_DRIFT1 = 4 
 // an x-line to the same HBM channels; a small pad spreads them.
 template <int N> __device__ __forceinline__ long zline_off(long l, long ps) {
     return (l / N) * ps + (l % N) * (long)N;
+}
+
+// Offset of half-spectrum row l = x N + y (rows of HS cells, planes sps apart)
+template <int N, int HS> __device__ __forceinline__ long hline_off(long l, long sps) {
+    return (l / N) * sps + (l % N) * (long)HS;
 }
 
 template <typename T, int N> constexpr int z_lines() {
@@ -163,7 +170,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
     load_twiddles<T, N>(tb, t, tw);
 
     if constexpr (MODE == Z_C2R) {
-        const cx<T> *row = shat + line * HS;
+        const cx<T> *row = shat + hline_off<N, HS>(line, za.sps);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int p = t + e * P;
@@ -198,7 +205,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         for (int e = 0; e < E; ++e) s[e] = mkc<T>(valid ? (T)(za.ms * (double)rrow[t + e * P]) : (T)0, (T)0);
         fft_line<T, N, -1, wave_line<T, N>()>(s, L, t, tb);
         if (valid) {
-            cx<T> *srow = shat + line * HS;
+            cx<T> *srow = shat + hline_off<N, HS>(line, za.sps);
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int p = t + e * P;
@@ -250,7 +257,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             }
             fft_line<T, N, -1, wave_line<T, N>()>(v, L, t, tb);
             if (valid) {
-                cx<T> *srow = shat + line * HS;
+                cx<T> *srow = shat + hline_off<N, HS>(line, za.sps);
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     const int p = t + e * P;
@@ -292,7 +299,7 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
     cx<T> *L = lds + l * LS;
     TwBase<T, H> tb;
     load_twiddles<T, H, 2>(tb, t, tw);
-    const cx<T> *row = shat + line * HS;
+    const cx<T> *row = shat + hline_off<N, HS>(line, za.sps);
     const cx<T> wt = tw[t];                                    // W^t; W^(t+eP) = W^t W^(eP)
     cx<T> v[E];
 #pragma unroll
@@ -401,9 +408,15 @@ template <typename T, int N> struct TileIO {
     }
 };
 
+// Minimum waves per SIMD asked of the compiler for a strided pass: 8 caps
+// the complex-float Green pass at 64 VGPRs, so two 1024-thread work-groups
+// share a CU (measured at 512^3: 0.39 -> 0.35 ms, one 8-byte spill); the
+// other passes fit without a cap (capping the z-pass C spilled and slowed).
+template <typename T, int MODE> constexpr int sp_wpe() { return (MODE == S_GREEN && sizeof(T) == 4) ? 8 : 1; }
+
 // one tile per work-group: grid (z-column tiles, lines of the fixed axis)
 template <typename T, int N, int AXIS, int MODE, bool WIDE>
-__global__ __launch_bounds__((s_cols<T, N, WIDE>() * (N / SchedT<T, N>::E)))
+__global__ __launch_bounds__((s_cols<T, N, WIDE>() * (N / SchedT<T, N>::E)), (sp_wpe<T, MODE>()))
 void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, SArgs sa) {
     constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE>(), LS = lds_line<N>();
     __shared__ cx<T> lds[TC * LS];
@@ -580,7 +593,7 @@ __global__ __launch_bounds__(256) void k_adj_kick(cx<T> *__restrict__ psi, cx<T>
 // kz = 0 and kz = N/2 planes and 2 elsewhere (Hermitian partners).  Each
 // block bins into LDS, then adds its bins to the global sums.
 template <typename T, int N>
-__global__ __launch_bounds__(256) void k_pk_bin(const cx<T> *__restrict__ shat, int hs, int nbins,
+__global__ __launch_bounds__(256) void k_pk_bin(const cx<T> *__restrict__ shat, int hs, long sps, int nbins,
                                                 double *__restrict__ out) {
     extern __shared__ double sb[];   // 3 * nbins
     for (int i = threadIdx.x; i < 3 * nbins; i += blockDim.x) sb[i] = 0.0;
@@ -594,7 +607,7 @@ __global__ __launch_bounds__(256) void k_pk_bin(const cx<T> *__restrict__ shat, 
         const double nn = sqrt((double)nx * nx + (double)ny * ny + (double)kz * kz);
         const int b = (int)(nn + 0.5);
         if (b < 1 || b > nbins) continue;
-        const cx<T> v = shat[r * hs + kz];
+        const cx<T> v = shat[(long)i * sps + (long)j * hs + kz];
         const double w = (kz == 0 || kz == N / 2) ? 1.0 : 2.0;
         atomicAdd(&sb[3 * (b - 1)], w * ((double)v.x * v.x + (double)v.y * v.y));
         atomicAdd(&sb[3 * (b - 1) + 1], w * nn);
