@@ -408,6 +408,20 @@ template <typename T, int N> struct TileIO {
     }
 };
 
+// Work-groups start round-robin over the 8 XCDs (linear id mod 8), each
+// with its own L2.  The kick pass's column tiles 2j and 2j+1 each read half
+// of every 128-byte segment of phi; this maps them to linear ids 8 apart,
+// i.e. to one XCD at about the same time, so the second half is an L2 hit.
+// Within each run of 16 linear ids: id 16g + 8h + x -> tile 16g + 2x + h.
+// Needs gridDim.x (tiles per line) to be a multiple of 16.
+__device__ __forceinline__ void xcd_pair(int &bx, int &by) {
+    const int L = blockIdx.x + gridDim.x * blockIdx.y;
+    const int r = L & 15;
+    const int Ll = (L & ~15) + 2 * (r & 7) + (r >> 3);
+    by = Ll / (int)gridDim.x;
+    bx = Ll - by * (int)gridDim.x;
+}
+
 // Minimum waves per SIMD asked of the compiler for a strided pass: 8 caps
 // the complex-float Green pass at 64 VGPRs, so two 1024-thread work-groups
 // share a CU (measured at 512^3: 0.39 -> 0.35 ms, one 8-byte spill); the
@@ -421,8 +435,10 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
     constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE>(), LS = lds_line<N>();
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
-    const int k0 = blockIdx.x * TC, k = k0 + c;
-    const int other = sa.other0 + block_order(sa.rev, blockIdx.y, gridDim.y);
+    int bx = blockIdx.x, by = blockIdx.y;
+    if constexpr (MODE == S_KICK_DRIFT1 && (N / TC) % 16 == 0) xcd_pair(bx, by);
+    const int k0 = bx * TC, k = k0 + c;
+    const int other = sa.other0 + block_order(sa.rev, by, gridDim.y);
     const bool valid = k < kv;
     long base0, stride;
     spass_addr<N, AXIS>(other, k0, rl, ps, base0, stride);
