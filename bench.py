@@ -90,12 +90,14 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-def timed_adaptive(F, cfg, psi, warmup: int, steps: int, prof_steps: int, keep_state: bool):
+def timed_adaptive(F, cfg, psi, warmup: int, steps: int, prof_steps: int, keep_state: bool, persistent: int = -1):
     """warm-up, then `steps` timed adaptive steps (ifdm_simulate, no events),
     then `prof_steps` more with live per-launch HIP events for the per-pass
     times.  Returns (steps/s, seconds, avg ms per pass, schedule of the
     warm-up + timed steps, psi and phi after them (or None), a)."""
     s = F.Solver(cfg)
+    if persistent >= 0:
+        s.set_option("persistent", persistent)
     s.set_psi(psi)
     a, sched_w = s.simulate_native(1.0 / 51, warmup)
     t0 = time.perf_counter()
@@ -326,6 +328,8 @@ def main():
                     help="also run the adjoint study over this many steps (0: skip)")
     ap.add_argument("--adjoint-k", type=int, default=50, help="checkpoint interval of the adjoint study")
     ap.add_argument("--adjoint-fd", type=int, default=2, help="finite-difference directions of the adjoint study")
+    ap.add_argument("--persistent", type=int, default=-1,
+                    help="persistent strided-pass mask for both precisions (-1: the library's defaults)")
     ap.add_argument("--plane-pad", type=int, default=0,
                     help="device cells after each x-plane of psi and phi (0: the library's choice, <0: none)")
     args = ap.parse_args()
@@ -361,7 +365,8 @@ def main():
 
     sps, el, avg, sched, psi64, phi64, a = timed_adaptive(F, cfg, psi, args.warmup, args.steps,
                                                           args.prof_steps,
-                                                          not (args.no_mixed or args.no_mixed_check))
+                                                          not (args.no_mixed or args.no_mixed_check),
+                                                          args.persistent)
     log(f"[bench] FP64: {args.steps} steps in {el:.3f} s ({sps:.2f} steps/s), a = {a:.5f}")
     pb = pass_bytes(n)
     b_alg = sum(pb)
@@ -391,7 +396,7 @@ def main():
     if not args.no_mixed:
         cfgm = F.Config.from_cli(2.5e-24, g_in, xmax_in, n, precision=F.MIXED, plane_pad=args.plane_pad)
         spsm, elm, avgm, _schm, _p, _f, am = timed_adaptive(F, cfgm, psi, args.warmup, args.steps,
-                                                            args.prof_steps, False)
+                                                            args.prof_steps, False, args.persistent)
         log(f"[bench] mixed: {args.steps} steps in {elm:.3f} s ({spsm:.2f} steps/s)")
         pbm = pass_bytes(n, 8, 4)
         out["mixed"] = {
