@@ -22,7 +22,8 @@ struct ifdm_ctx {
     void *psi = nullptr;       // N^3 complex (plane stride ps)
     void *phi = nullptr;       // N^3 real (plane stride ps)
     void *shat = nullptr;      // N planes of N*HS complex (plane stride sps)
-    void *tw = nullptr;        // N complex twiddles
+    void *tw = nullptr;        // N complex twiddles (storage precision)
+    void *tw64 = nullptr;      // N complex FP64 twiddles (the same buffer in FP64 mode)
     double *k2 = nullptr;      // N doubles
     double *part_rho = nullptr;
     double *part_phi = nullptr;

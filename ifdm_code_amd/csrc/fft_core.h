@@ -90,14 +90,24 @@ template <int DIR, typename T> __device__ __forceinline__ void bfly4(cx<T> &a0, 
     a1 = cadd(b1, b3);
     a3 = csub(b1, b3);
 }
-template <int DIR, typename T> __device__ __forceinline__ void bfly8(cx<T> *a) {
+// x / sqrt(2); ACC (complex floats): the constant as a sum of two floats,
+// since a rounded 1/sqrt(2) alone scales |W8^1|, |W8^3| by 1 - 3.4e-8
+template <bool ACC, typename T> __device__ __forceinline__ T mul_rsqrt2(T x) {
+    if constexpr (ACC && sizeof(T) == 4) {
+        constexpr float hi = 0.707106769084930419921875f;        // float(1/sqrt 2)
+        constexpr float lo = 1.21016175e-8f;                     // 1/sqrt 2 - hi
+        return fmaf(hi, x, lo * x);
+    } else {
+        return (T)0.707106781186547524400844362104849 * x;
+    }
+}
+template <int DIR, bool ACC = false, typename T> __device__ __forceinline__ void bfly8(cx<T> *a) {
     bfly4<DIR>(a[0], a[2], a[4], a[6]);
     bfly4<DIR>(a[1], a[3], a[5], a[7]);
-    const T h = (T)0.707106781186547524400844362104849;
     cx<T> o1 = a[3], o3 = a[7];
-    a[3] = mkc<T>(h * (o1.x - DIR * o1.y), h * (o1.y + DIR * o1.x));     // * W8^1
-    a[5] = crot<DIR>(a[5]);                                               // * W8^2
-    a[7] = mkc<T>(h * (-o3.x - DIR * o3.y), h * (-o3.y + DIR * o3.x));   // * W8^3
+    a[3] = mkc<T>(mul_rsqrt2<ACC>(o1.x - DIR * o1.y), mul_rsqrt2<ACC>(o1.y + DIR * o1.x));     // * W8^1
+    a[5] = crot<DIR>(a[5]);                                                                     // * W8^2
+    a[7] = mkc<T>(mul_rsqrt2<ACC>(-o3.x - DIR * o3.y), mul_rsqrt2<ACC>(-o3.y + DIR * o3.x));   // * W8^3
     cx<T> e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
     cx<T> p0 = a[1], p1 = a[3], p2 = a[5], p3 = a[7];
     a[0] = cadd(e0, p0); a[4] = csub(e0, p0);
@@ -105,10 +115,10 @@ template <int DIR, typename T> __device__ __forceinline__ void bfly8(cx<T> *a) {
     a[2] = cadd(e2, p2); a[6] = csub(e2, p2);
     a[3] = cadd(e3, p3); a[7] = csub(e3, p3);
 }
-template <int R, int DIR, typename T> __device__ __forceinline__ void dft(cx<T> *a) {
+template <int R, int DIR, bool ACC = false, typename T> __device__ __forceinline__ void dft(cx<T> *a) {
     if constexpr (R == 2) bfly2<DIR>(a[0], a[1]);
     else if constexpr (R == 4) bfly4<DIR>(a[0], a[1], a[2], a[3]);
-    else if constexpr (R == 8) bfly8<DIR>(a);
+    else if constexpr (R == 8) bfly8<DIR, ACC>(a);
 }
 
 // ---------------------------------------------------------------- line FFT
@@ -118,16 +128,18 @@ template <int R, int DIR, typename T> __device__ __forceinline__ void dft(cx<T> 
 // 3: w^2 w; 4: (w^2)^2; 5: w^4 w; 6: w^4 w^2; 7: w^4 w^3).  The base
 // values are loaded once per thread at the start of a pass, so their
 // latency hides behind the data loads instead of stalling each stage.
-template <typename T, int N> struct TwBase {
+// W: the type of the base values (FP64 for complex floats where the
+// twiddles must be accurate, see apply_twiddles)
+template <typename T, int N, typename W = T> struct TwBase {
     static constexpr int E = SchedT<T, N>::E;
-    cx<T> w[4][E];  // [stage][butterfly]; stage 0 unused
+    cx<W> w[4][E];  // [stage][butterfly]; stage 0 unused
 };
 
 // tw is the table W_M^k, k < M, of a transform of length M = TS * N
 // (TS = 1: the line's own table; TS = 2: a half-length transform reading the
 // full-length table)
-template <typename T, int N, int TS = 1>
-__device__ __forceinline__ void load_twiddles(TwBase<T, N> &b, int t, const cx<T> *__restrict__ tw) {
+template <typename T, int N, int TS = 1, typename W>
+__device__ __forceinline__ void load_twiddles(TwBase<T, N, W> &b, int t, const cx<W> *__restrict__ tw) {
     constexpr int E = SchedT<T, N>::E, P = N / E;
 #pragma unroll
     for (int S = 1; S < SchedT<T, N>::ns; ++S) {
@@ -137,8 +149,59 @@ __device__ __forceinline__ void load_twiddles(TwBase<T, N> &b, int t, const cx<T
     }
 }
 
-template <int R, int DIR, typename T>
-__device__ __forceinline__ void apply_twiddles(cx<T> *a, cx<T> w1) {
+// Complex floats, ACC: the powers of the FP64 base twiddle are formed in
+// FP64 and each product a * w is taken in FP64 and rounded once.  Rounded
+// FP32 twiddles carry a fixed error per mode that the kick-drift sequence
+// compounds step after step (a 7e-4 density error after 1000 steps at
+// 512^3, against 2e-6 from FP32 storage alone).
+__device__ __forceinline__ cx<float> twmul64(cx<float> a, cx<double> w) {
+    const double ax = a.x, ay = a.y;
+    return mkc<float>((float)(ax * w.x - ay * w.y), (float)(ax * w.y + ay * w.x));
+}
+template <int R, int DIR, bool ACC>
+__device__ __forceinline__ void apply_twiddles(cx<float> *a, cx<float> w1) {
+    using T = float;
+    if constexpr (DIR > 0) w1.y = -w1.y;
+    a[1] = cmul(a[1], w1);
+    if constexpr (R >= 4) {
+        const cx<T> w2 = cmul(w1, w1);
+        a[2] = cmul(a[2], w2);
+        a[3] = cmul(a[3], cmul(w2, w1));
+        if constexpr (R == 8) {
+            const cx<T> w4 = cmul(w2, w2), w3 = cmul(w2, w1);
+            a[4] = cmul(a[4], w4);
+            a[5] = cmul(a[5], cmul(w4, w1));
+            a[6] = cmul(a[6], cmul(w4, w2));
+            a[7] = cmul(a[7], cmul(w4, w3));
+        }
+    }
+}
+
+template <int R, int DIR, bool ACC>
+__device__ __forceinline__ void apply_twiddles(cx<float> *a, cx<double> w) {
+    if constexpr (!ACC) {
+        apply_twiddles<R, DIR, false>(a, mkc<float>((float)w.x, (float)w.y));
+    } else {
+        if constexpr (DIR > 0) w.y = -w.y;
+        a[1] = twmul64(a[1], w);
+        if constexpr (R >= 4) {
+            const cx<double> w2 = cmul(w, w);
+            a[2] = twmul64(a[2], w2);
+            a[3] = twmul64(a[3], cmul(w2, w));
+            if constexpr (R == 8) {
+                const cx<double> w4 = cmul(w2, w2), w3 = cmul(w2, w);
+                a[4] = twmul64(a[4], w4);
+                a[5] = twmul64(a[5], cmul(w4, w));
+                a[6] = twmul64(a[6], cmul(w4, w2));
+                a[7] = twmul64(a[7], cmul(w4, w3));
+            }
+        }
+    }
+}
+
+template <int R, int DIR, bool ACC>
+__device__ __forceinline__ void apply_twiddles(cx<double> *a, cx<double> w1) {
+    using T = double;
     if constexpr (DIR > 0) w1.y = -w1.y;
     a[1] = cmul(a[1], w1);
     if constexpr (R >= 4) {
@@ -175,8 +238,8 @@ template <bool WAVE> __device__ __forceinline__ void line_sync() {
 // +1 inverse), for the thread at line position t.  L is the line's LDS
 // region; b the thread's base twiddles.  FRESH: the work-group has not yet
 // used its LDS, so the first scatter needs no preceding synchronisation.
-template <typename T, int N, int DIR, int S, bool WAVE = false, bool FRESH = false>
-__device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &b) {
+template <typename T, int N, int DIR, int S, bool WAVE = false, bool FRESH = false, bool ACC = false, typename W>
+__device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const TwBase<T, N, W> &b) {
     constexpr int E = SchedT<T, N>::E, P = N / E, R = SchedT<T, N>::r[S], Q = E / R;
     constexpr int NS = ns_before<N, E>(S);
     constexpr bool FIRST = (S == 0), LAST = (S == SchedT<T, N>::ns - 1);
@@ -189,8 +252,8 @@ __device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const TwBas
             if constexpr (FIRST) a[q][r] = v[q + r * Q];
             else a[q][r] = L[lpos(j + r * (N / R))];
         }
-        if constexpr (!FIRST) apply_twiddles<R, DIR>(a[q], b.w[S][q]);
-        dft<R, DIR>(a[q]);
+        if constexpr (!FIRST) apply_twiddles<R, DIR, ACC>(a[q], b.w[S][q]);
+        dft<R, DIR, ACC>(a[q]);
     }
     if constexpr (LAST) {
 #pragma unroll
@@ -210,12 +273,24 @@ __device__ __forceinline__ void fft_stage(cx<T> *v, cx<T> *L, int t, const TwBas
     }
 }
 
-template <typename T, int N, int DIR, bool WAVE = false, bool FRESH = false>
-__device__ __forceinline__ void fft_line(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &b) {
-    fft_stage<T, N, DIR, 0, WAVE, FRESH>(v, L, t, b);
-    if constexpr (SchedT<T, N>::ns > 1) fft_stage<T, N, DIR, 1, WAVE>(v, L, t, b);
-    if constexpr (SchedT<T, N>::ns > 2) fft_stage<T, N, DIR, 2, WAVE>(v, L, t, b);
-    if constexpr (SchedT<T, N>::ns > 3) fft_stage<T, N, DIR, 3, WAVE>(v, L, t, b);
+// ACC: accurate twiddles for complex floats (see apply_twiddles)
+template <typename T, int N, int DIR, bool WAVE = false, bool FRESH = false, bool ACC = false, typename W>
+__device__ __forceinline__ void fft_line(cx<T> *v, cx<T> *L, int t, const TwBase<T, N, W> &b0) {
+    // FP64 bases of complex-float transforms pass an empty asm per
+    // transform, so the compiler forms their powers again instead of keeping
+    // a pass's earlier powers live in registers across its other transforms
+    TwBase<T, N, W> b = b0;
+    if constexpr (sizeof(T) == 4 && sizeof(W) == 8) {
+#pragma unroll
+        for (int S = 1; S < SchedT<T, N>::ns; ++S)
+#pragma unroll
+            for (int q = 0; q < SchedT<T, N>::E / SchedT<T, N>::r[S]; ++q)
+                asm volatile("" : "+v"(b.w[S][q].x), "+v"(b.w[S][q].y));
+    }
+    fft_stage<T, N, DIR, 0, WAVE, FRESH, ACC>(v, L, t, b);
+    if constexpr (SchedT<T, N>::ns > 1) fft_stage<T, N, DIR, 1, WAVE, false, ACC>(v, L, t, b);
+    if constexpr (SchedT<T, N>::ns > 2) fft_stage<T, N, DIR, 2, WAVE, false, ACC>(v, L, t, b);
+    if constexpr (SchedT<T, N>::ns > 3) fft_stage<T, N, DIR, 3, WAVE, false, ACC>(v, L, t, b);
 }
 
 // convenience: load the base twiddles and transform

@@ -67,7 +67,7 @@ struct Ops {
         int rc;
         if ((rc = prof_begin(c))) return rc;
         hipLaunchKernelGGL((k_zpass<T, N, MODE>), dim3(nb), dim3(LZ * P), 0, c->stream,
-                           (cx<T> *)c->psi, (T *)c->phi, (cx<T> *)c->shat, (const cx<T> *)c->tw, za, part);
+                           (cx<T> *)c->psi, (T *)c->phi, (cx<T> *)c->shat, (const cx<T> *)c->tw, (const cx<double> *)c->tw64, za, part);
         LAUNCHCHK(c);
         return prof_end(c);
     }
@@ -86,7 +86,7 @@ struct Ops {
             int rc;
             if ((rc = prof_begin(c))) return rc;
             hipLaunchKernelGGL((k_c2r_half<T, N>), dim3(nb), dim3(z_lines<T, N>() * P), 0, c->stream,
-                               (const cx<T> *)c->shat, (T *)c->phi, (const cx<T> *)c->tw, za, c->part_phi);
+                               (const cx<T> *)c->shat, (T *)c->phi, (const cx<T> *)c->tw, (const cx<double> *)c->tw64, za, c->part_phi);
             LAUNCHCHK(c);
             return prof_end(c);
         }
@@ -122,14 +122,14 @@ struct Ops {
             const int ntiles = ntz * no;
             const int nb = std::min(ntiles, occ * c->num_cu);
             hipLaunchKernelGGL((k_spass_pp<T, N, AXIS, MODE>), dim3(nb), dim3(TC * P), 0, c->stream,
-                               (cx<T> *)arr, rl, ps, kv, (const cx<T> *)c->tw, sa, ntz, ntiles);
+                               (cx<T> *)arr, rl, ps, kv, (const cx<T> *)c->tw, (const cx<double> *)c->tw64, sa, ntz, ntiles);
           }
         }
         if (!done) {
             constexpr int TC = s_cols<T, N, WIDE>();
             const int ntz = (kv + TC - 1) / TC;
             hipLaunchKernelGGL((k_spass<T, N, AXIS, MODE, WIDE>), dim3(ntz, no), dim3(TC * P), 0, c->stream,
-                               (cx<T> *)arr, rl, ps, kv, (const cx<T> *)c->tw, sa);
+                               (cx<T> *)arr, rl, ps, kv, (const cx<T> *)c->tw, (const cx<double> *)c->tw64, sa);
         }
         LAUNCHCHK(c);
         return prof_end(c);
@@ -405,6 +405,7 @@ void release(ifdm_ctx *c) {
                     c->adj_g, c->adj_s, c->adj_phs, c->ck};
     for (void *b : bufs)
         if (b) hipFree(b);
+    if (c->tw64 && c->tw64 != c->tw) hipFree(c->tw64);
     if (c->red_host) hipHostFree(c->red_host);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -509,6 +510,13 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     host_twiddles(n, tw, twf);
     for (int k = 0; k < n; ++k) k2[k] = std::pow(kvec(k, n, cfg->xmax), 2);
     const void *twsrc = c->mixed ? (const void *)twf.data() : (const void *)tw.data();
+    if (!c->mixed) {
+        c->tw64 = c->tw;
+    } else if ((e = hipMalloc(&c->tw64, 2 * ds * n)) != hipSuccess ||
+               (e = hipMemcpy(c->tw64, tw.data(), 2 * ds * n, hipMemcpyHostToDevice)) != hipSuccess) {
+        c->tw64 = nullptr;
+        return fail(IFDM_E_HIP, std::string("twiddles: ") + hipGetErrorString(e));
+    }
     if ((e = hipMemcpy(c->tw, twsrc, cs * n, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(c->k2, k2.data(), ds * n, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemset(c->psi, 0, cs * (size_t)c->np)) != hipSuccess ||
@@ -766,6 +774,7 @@ int ifdm_internal_theta_solve(ifdm_ctx *c, void *buf64, double coef) {
     host_twiddles(c->n, tw, twf);
     HIPCHK(c, hipMalloc(&sh.tw, tw.size() * sizeof(double)));
     int rc = IFDM_OK;
+    sh.tw64 = sh.tw;
     if (hipMemcpy(sh.tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = IFDM_E_HIP;
     if (!rc) rc = d.theta_solve(&sh, coef);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = IFDM_E_HIP;

@@ -23,6 +23,8 @@
 // poisson() at :80-119; its drift multiplies by fftnorm at :141 and its
 // Poisson solve by fftnorm after the c2r at :116.
 #pragma once
+#include <type_traits>
+
 #include "fastmath.h"
 #include "fft_core.h"
 
@@ -152,11 +154,22 @@ __device__ __forceinline__ double block_reduce(double v, double *sbuf, int tid, 
     return r;
 }
 
+// Complex floats: the drift's round trips on psi (passes A, B and the z
+// round trip of C; also the 3-D transforms of psi) take accurate twiddles
+// (FP64 base values, powers and products); the Poisson transforms keep FP32
+// twiddles.  Measured at 512^3 (density error after 1000 steps against the
+// FP64 path): FP32 twiddles everywhere 1.0e-3, accurate twiddles in the
+// Poisson transforms only 1.0e-3, in the psi round trips only 1.1e-5, in
+// both 1.1e-5.  The psi round trips repeat the same transform pair every
+// step on the field that persists; an error in the potential does not
+// carry over to the next step's.
+template <typename T, bool PSI> using tw_base_t = typename std::conditional<(sizeof(T) == 4 && PSI), double, T>::type;
+
 // ------------------------------------------------------------------ z passes
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__((z_lines<T, N>() * (N / SchedT<T, N>::E)))
 void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ shat,
-             const cx<T> *__restrict__ tw, ZArgs za, double *__restrict__ part) {
+             const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, ZArgs za, double *__restrict__ part) {
     constexpr int E = SchedT<T, N>::E, P = N / E, LZ = z_lines<T, N>(), LS = lds_line<N>();
     constexpr int HS = hs_of(N), NT = LZ * P;
     __shared__ cx<T> lds[LZ * LS];
@@ -166,8 +179,10 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
     const bool valid = line < (long)N * N;
     cx<T> *L = lds + l * LS;
     cx<T> v[E];
-    TwBase<T, N> tb;
-    load_twiddles<T, N>(tb, t, tw);
+    constexpr bool PSI = MODE == Z_DRIFT || MODE == Z_DRIFT_SRC || MODE == Z_FWD || MODE == Z_INV;
+    TwBase<T, N, tw_base_t<T, PSI>> tb;
+    if constexpr (sizeof(T) == 4 && PSI) load_twiddles<T, N>(tb, t, tw64);
+    else load_twiddles<T, N>(tb, t, tw);
 
     if constexpr (MODE == Z_C2R) {
         const cx<T> *row = shat + hline_off<N, HS>(line, za.sps);
@@ -181,7 +196,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             }
             v[e] = s;
         }
-        fft_line<T, N, +1, wave_line<T, N>()>(v, L, t, tb);
+        fft_line<T, N, +1, wave_line<T, N>(), false, false>(v, L, t, tb);
         double mx = 0.0;
         T *out = phi + zline_off<N>(line, za.ps);
 #pragma unroll
@@ -203,7 +218,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         cx<T> s[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) s[e] = mkc<T>(valid ? (T)(za.ms * (double)rrow[t + e * P]) : (T)0, (T)0);
-        fft_line<T, N, -1, wave_line<T, N>()>(s, L, t, tb);
+        fft_line<T, N, -1, wave_line<T, N>(), false, false>(s, L, t, tb);
         if (valid) {
             cx<T> *srow = shat + hline_off<N, HS>(line, za.sps);
 #pragma unroll
@@ -221,10 +236,10 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         if constexpr (MODE == Z_DRIFT || MODE == Z_DRIFT_SRC) {
             // the z factor of the drift: forward, E[k_z] / N, inverse
             const cx<T> *tab = (const cx<T> *)za.dtab;
-            fft_line<T, N, -1, wave_line<T, N>()>(v, L, t, tb);
+            fft_line<T, N, -1, wave_line<T, N>(), false, (sizeof(T) == 4)>(v, L, t, tb);
 #pragma unroll
             for (int e = 0; e < E; ++e) v[e] = cscale(cmul(v[e], tab[t + e * P]), (T)(1.0 / N));
-            fft_line<T, N, +1, wave_line<T, N>()>(v, L, t, tb);
+            fft_line<T, N, +1, wave_line<T, N>(), false, (sizeof(T) == 4)>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -232,7 +247,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             if constexpr (MODE == Z_DRIFT) return;
         }
         if constexpr (MODE == Z_FWD) {
-            fft_line<T, N, -1, wave_line<T, N>()>(v, L, t, tb);
+            fft_line<T, N, -1, wave_line<T, N>(), false, (sizeof(T) == 4)>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -240,7 +255,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             return;
         }
         if constexpr (MODE == Z_INV) {
-            fft_line<T, N, +1, wave_line<T, N>()>(v, L, t, tb);
+            fft_line<T, N, +1, wave_line<T, N>(), false, (sizeof(T) == 4)>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -255,7 +270,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
                 if (valid) { mx = fmax(mx, rr); sm += rr; }
                 v[e] = mkc<T>(valid ? (T)(za.ms * rr - za.rhoc) : (T)0, (T)0);
             }
-            fft_line<T, N, -1, wave_line<T, N>()>(v, L, t, tb);
+            fft_line<T, N, -1, wave_line<T, N>(), false, false>(v, L, t, tb);
             if (valid) {
                 cx<T> *srow = shat + hline_off<N, HS>(line, za.sps);
 #pragma unroll
@@ -286,7 +301,7 @@ template <typename T, int N> constexpr int c2r_lines() {
 
 template <typename T, int N>
 __global__ __launch_bounds__((z_lines<T, N>() * (N / SchedT<T, N>::E)))
-void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T> *__restrict__ tw, ZArgs za,
+void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, ZArgs za,
                 double *__restrict__ part) {
     constexpr int H = N / 2, E = SchedT<T, H>::E, P = H / E, LZ = c2r_lines<T, N>(), LS = lds_line<H>();
     constexpr int HS = hs_of(N), NT = LZ * P;
@@ -318,7 +333,7 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
         dif = cmul(dif, w);
         v[e] = mkc<T>(sum.x - dif.y, sum.y + dif.x);           // sum + i dif
     }
-    fft_line<T, H, +1, wave_line<T, H>()>(v, L, t, tb);
+    fft_line<T, H, +1, wave_line<T, H>(), false, false>(v, L, t, tb);
     double mx = 0.0;
     cx<T> *out = reinterpret_cast<cx<T> *>(phi + zline_off<N>(line, za.ps));
 #pragma unroll
@@ -353,15 +368,16 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
 // the transform (and the spectral multiply of passes C and G) of one tile;
 // FRESH: the first transform of the work-group (no earlier LDS use).  tab
 // and k2 are the drift table and k_i^2 (global memory, or LDS copies).
-template <typename T, int N, int MODE, bool FRESH>
-__device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const TwBase<T, N> &tw,
+template <typename T, int N, int MODE, bool FRESH, int AXIS, typename W>
+__device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const TwBase<T, N, W> &tw,
                                               const SArgs &sa, int other, int k, bool valid,
                                               const cx<T> *tab, const double *k2) {
     constexpr int E = SchedT<T, N>::E, P = N / E;
-    if constexpr (MODE == S_FWD) fft_line<T, N, -1, false, FRESH>(v, L, t, tw);
-    else if constexpr (MODE == S_INV) fft_line<T, N, +1, false, FRESH>(v, L, t, tw);
+    constexpr bool ACC = sizeof(T) == 4 && (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1);
+    if constexpr (MODE == S_FWD) fft_line<T, N, -1, false, FRESH, ACC>(v, L, t, tw);
+    else if constexpr (MODE == S_INV) fft_line<T, N, +1, false, FRESH, ACC>(v, L, t, tw);
     else {
-        fft_line<T, N, -1, false, FRESH>(v, L, t, tw);
+        fft_line<T, N, -1, false, FRESH, ACC>(v, L, t, tw);
         // spectral indices of this thread's points: along the line i, fixed `other` and k
         if constexpr (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1) {
 #pragma unroll
@@ -375,7 +391,7 @@ __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const T
                 v[e] = cscale(v[e], (T)gf);
             }
         }
-        fft_line<T, N, +1>(v, L, t, tw);
+        fft_line<T, N, +1, false, false, ACC>(v, L, t, tw);
     }
 }
 
@@ -431,7 +447,7 @@ template <typename T, int MODE> constexpr int sp_wpe() { return (MODE == S_GREEN
 // one tile per work-group: grid (z-column tiles, lines of the fixed axis)
 template <typename T, int N, int AXIS, int MODE, bool WIDE>
 __global__ __launch_bounds__((s_cols<T, N, WIDE>() * (N / SchedT<T, N>::E)), (sp_wpe<T, MODE>()))
-void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, SArgs sa) {
+void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa) {
     constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE>(), LS = lds_line<N>();
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
@@ -444,8 +460,10 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
     spass_addr<N, AXIS>(other, k0, rl, ps, base0, stride);
     const TileIO<T, N> io(a, base0);
     cx<T> *L = lds + c * LS;
-    TwBase<T, N> tb;
-    load_twiddles<T, N>(tb, t, tw);
+    constexpr bool PSI = MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1;
+    TwBase<T, N, tw_base_t<T, PSI>> tb;
+    if constexpr (sizeof(T) == 4 && PSI) load_twiddles<T, N>(tb, t, tw64);
+    else load_twiddles<T, N>(tb, t, tw);
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = valid ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
@@ -458,7 +476,7 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
             v[e] = cmul(v[e], expmi<T>(sa.aphi * f + sa.arho * rr));
         }
     }
-    spass_compute<T, N, MODE, true>(v, L, t, tb, sa, other, k, valid, (const cx<T> *)sa.dtab, sa.k2);
+    spass_compute<T, N, MODE, true, AXIS>(v, L, t, tb, sa, other, k, valid, (const cx<T> *)sa.dtab, sa.k2);
     if (valid) {
 #pragma unroll
         for (int e = 0; e < E; ++e) io.store(c + (long)(t + e * P) * stride, v[e]);
@@ -472,7 +490,7 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
 // transform is preceded by a barrier, so consecutive tiles never race on LDS.
 template <typename T, int N, int AXIS, int MODE>
 __global__ __launch_bounds__((s_cols<T, N>() * (N / SchedT<T, N>::E)))
-void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, SArgs sa, int ntz,
+void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa, int ntz,
                 int ntiles) {
     constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N>(), LS = lds_line<N>();
     constexpr int NT = TC * P;
@@ -505,8 +523,10 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__r
     // the count of memory operations in flight is the same on every path:
     // the wait for the prefetch then lets this tile's stores stay in flight.
     constexpr bool FULL = N >= TC;
-    TwBase<T, N> tb;
-    load_twiddles<T, N>(tb, t, tw);
+    constexpr bool PSI = MODE == S_DRIFT1;
+    TwBase<T, N, tw_base_t<T, PSI>> tb;
+    if constexpr (sizeof(T) == 4 && PSI) load_twiddles<T, N>(tb, t, tw64);
+    else load_twiddles<T, N>(tb, t, tw);
     cx<T> v[E], nx[E];
     {
         const int other = sa.other0 + tile / ntz, k0 = (tile % ntz) * TC;
@@ -539,7 +559,7 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__r
         __builtin_amdgcn_sched_barrier(0);   // issue the prefetch before the transform
         const int other = sa.other0 + tile / ntz, k0 = (tile % ntz) * TC, k = k0 + c;
         const bool valid = k < kv;
-        spass_compute<T, N, MODE, false>(v, L, t, tb, sa, other, k, valid, tab, k2t);
+        spass_compute<T, N, MODE, false, AXIS>(v, L, t, tb, sa, other, k, valid, tab, k2t);
         {
             long b0, st;
             spass_addr<N, AXIS>(other, k0, rl, ps, b0, st);
