@@ -162,8 +162,13 @@ __device__ __forceinline__ double block_reduce(double v, double *sbuf, int tid, 
 // Poisson transforms only 1.0e-3, in the psi round trips only 1.1e-5, in
 // both 1.1e-5.  The psi round trips repeat the same transform pair every
 // step on the field that persists; an error in the potential does not
-// carry over to the next step's.
-template <typename T, bool PSI> using tw_base_t = typename std::conditional<(sizeof(T) == 4 && PSI), double, T>::type;
+// carry over to the next step's.  Up to N = 512: the 16-point-per-thread
+// transforms of N = 1024 have no register room for FP64 twiddles (measured
+// 31 -> 25 steps/s at 1024^3), so they keep FP32 twiddles (the first 100
+// steps then stay near 1e-5, the error of FP32 twiddles at 512^3).
+template <typename T, int N> constexpr bool acc_tw() { return sizeof(T) == 4 && N <= 512; }
+template <typename T, int N, bool PSI>
+using tw_base_t = typename std::conditional<(acc_tw<T, N>() && PSI), double, T>::type;
 
 // ------------------------------------------------------------------ z passes
 template <typename T, int N, int MODE>
@@ -180,8 +185,8 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
     cx<T> *L = lds + l * LS;
     cx<T> v[E];
     constexpr bool PSI = MODE == Z_DRIFT || MODE == Z_DRIFT_SRC || MODE == Z_FWD || MODE == Z_INV;
-    TwBase<T, N, tw_base_t<T, PSI>> tb;
-    if constexpr (sizeof(T) == 4 && PSI) load_twiddles<T, N>(tb, t, tw64);
+    TwBase<T, N, tw_base_t<T, N, PSI>> tb;
+    if constexpr (acc_tw<T, N>() && PSI) load_twiddles<T, N>(tb, t, tw64);
     else load_twiddles<T, N>(tb, t, tw);
 
     if constexpr (MODE == Z_C2R) {
@@ -236,10 +241,10 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         if constexpr (MODE == Z_DRIFT || MODE == Z_DRIFT_SRC) {
             // the z factor of the drift: forward, E[k_z] / N, inverse
             const cx<T> *tab = (const cx<T> *)za.dtab;
-            fft_line<T, N, -1, wave_line<T, N>(), false, (sizeof(T) == 4)>(v, L, t, tb);
+            fft_line<T, N, -1, wave_line<T, N>(), false, acc_tw<T, N>()>(v, L, t, tb);
 #pragma unroll
             for (int e = 0; e < E; ++e) v[e] = cscale(cmul(v[e], tab[t + e * P]), (T)(1.0 / N));
-            fft_line<T, N, +1, wave_line<T, N>(), false, (sizeof(T) == 4)>(v, L, t, tb);
+            fft_line<T, N, +1, wave_line<T, N>(), false, acc_tw<T, N>()>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -247,7 +252,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             if constexpr (MODE == Z_DRIFT) return;
         }
         if constexpr (MODE == Z_FWD) {
-            fft_line<T, N, -1, wave_line<T, N>(), false, (sizeof(T) == 4)>(v, L, t, tb);
+            fft_line<T, N, -1, wave_line<T, N>(), false, acc_tw<T, N>()>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -255,7 +260,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             return;
         }
         if constexpr (MODE == Z_INV) {
-            fft_line<T, N, +1, wave_line<T, N>(), false, (sizeof(T) == 4)>(v, L, t, tb);
+            fft_line<T, N, +1, wave_line<T, N>(), false, acc_tw<T, N>()>(v, L, t, tb);
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < E; ++e) row[t + e * P] = v[e];
@@ -373,7 +378,7 @@ __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const T
                                               const SArgs &sa, int other, int k, bool valid,
                                               const cx<T> *tab, const double *k2) {
     constexpr int E = SchedT<T, N>::E, P = N / E;
-    constexpr bool ACC = sizeof(T) == 4 && (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1);
+    constexpr bool ACC = acc_tw<T, N>() && (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1);
     if constexpr (MODE == S_FWD) fft_line<T, N, -1, false, FRESH, ACC>(v, L, t, tw);
     else if constexpr (MODE == S_INV) fft_line<T, N, +1, false, FRESH, ACC>(v, L, t, tw);
     else {
@@ -461,8 +466,8 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
     const TileIO<T, N> io(a, base0);
     cx<T> *L = lds + c * LS;
     constexpr bool PSI = MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1;
-    TwBase<T, N, tw_base_t<T, PSI>> tb;
-    if constexpr (sizeof(T) == 4 && PSI) load_twiddles<T, N>(tb, t, tw64);
+    TwBase<T, N, tw_base_t<T, N, PSI>> tb;
+    if constexpr (acc_tw<T, N>() && PSI) load_twiddles<T, N>(tb, t, tw64);
     else load_twiddles<T, N>(tb, t, tw);
     cx<T> v[E];
 #pragma unroll
@@ -524,8 +529,8 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__r
     // the wait for the prefetch then lets this tile's stores stay in flight.
     constexpr bool FULL = N >= TC;
     constexpr bool PSI = MODE == S_DRIFT1;
-    TwBase<T, N, tw_base_t<T, PSI>> tb;
-    if constexpr (sizeof(T) == 4 && PSI) load_twiddles<T, N>(tb, t, tw64);
+    TwBase<T, N, tw_base_t<T, N, PSI>> tb;
+    if constexpr (acc_tw<T, N>() && PSI) load_twiddles<T, N>(tb, t, tw64);
     else load_twiddles<T, N>(tb, t, tw);
     cx<T> v[E], nx[E];
     {
