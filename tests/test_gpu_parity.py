@@ -378,6 +378,34 @@ def test_long_run_against_oracle_fixture():
             assert st["max_rho"] == pytest.approx(float(fx[f"maxrho{c}"]), rel=1e-10)
 
 
+def test_mixed_long_run_density_error():
+    """The mixed path over the 1000-step 128^3 schedule of the oracle
+    fixture against the FP64 path from the same IC: the density stays
+    within the stated 1e-4 (relative L2; the drift's transforms on psi use
+    accurate twiddles, without which the FP32 twiddles' fixed errors
+    compound over the steps)."""
+    import os
+    from ifdm_code_amd import icgen
+    F, _O = _mods()
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "long_128_oracle.npz"))
+    n = 128
+    sched = [tuple(x) for x in fx["sched"]]
+    _say(f"[mixed long run] n={n} steps={len(sched)}")
+    pos, vel, pm, a0 = icgen.zeldovich(n, 10.0, z=50.0, seed=1234)
+    with F.Solver(_cfg(F, n)) as s:
+        s.ic_from_particles([(pos, vel)], 10.0e3, pm * 1e10, a0)
+        psi0 = s.get_psi()
+    rho = {}
+    for prec in (F.FP64, F.MIXED):
+        with F.Solver(F.Config.from_cli(2.5e-24, 0.0, 5.0, n, precision=prec)) as s:
+            s.set_psi(psi0)
+            s.poisson()
+            s.run(sched)
+            rho[prec] = np.abs(s.get_psi()) ** 2
+    err = rel_l2(rho[F.MIXED], rho[F.FP64])
+    _say(f"   mixed density rel L2 after {len(sched)} steps = {err:.3e}")
+    assert err < 1e-4
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_plane_wave_drift_1024(precision):
     """The largest supported mesh, 1024^3, in both precisions: a plane wave
