@@ -141,7 +141,7 @@ struct Ops {
     // after three passes instead of five; the kick rides on the first of
     // them and the Poisson source with its z transform on the last.  One
     // pass of the step by index:
-    //   0 T  the per-axis drift table E[i] = exp(-i c k_i^2 dKE)
+    //   0 T  the per-axis drift table E[i] = exp(-i c k_i^2 dKE) / N
     //   1 A  y: kick (aphi, arho; both 0: none), forward, E[k_y]/N, inverse
     //   2 B  x: forward, E[k_x]/N, inverse
     //   3 C  z: forward, E[k_z]/N, inverse; source s = m_s|psi|^2 - rho_c, its

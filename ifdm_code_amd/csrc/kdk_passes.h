@@ -10,7 +10,7 @@
 // One step is a drift table and eight passes, each one read and one write
 // of a field (the drift factor separates by axis, so each axis makes its
 // own forward - factor - inverse round trip):
-//   T      drift table E[i] = exp(-i c k_i^2 dKE)
+//   T      drift table E[i] = exp(-i c k_i^2 dKE) / N
 //   A  y:  kick (phase from phi and |psi|^2), forward, E[k_y]/N, inverse  psi
 //   B  x:  forward, E[k_x]/N, inverse                                      psi
 //   C  z:  forward, E[k_z]/N, inverse -> psi; m_s|psi|^2 - rho_c -> real
@@ -124,11 +124,13 @@ template <typename T> __device__ __forceinline__ cx<T> expmi(double th) {
 }
 
 // The drift phase exp(-i KE(k) dKE), KE = ke_coef (kx^2 + ky^2 + kz^2), is a
-// product of one factor per axis.  One block of pass A writes the per-step
-// table E[i] = exp(-i ke_coef k_i^2 dKE), i < N, which pass C reads.
+// product of one factor per axis.  Pass T writes the per-step table
+// E[i] = exp(-i ke_coef k_i^2 dKE) / N, i < N, which each axis's round trip
+// multiplies in (the 1/N of its inverse transform folded in: a power of two,
+// so the products are the same bits as scaling afterwards).
 template <typename T, int N>
 __device__ __forceinline__ void write_drift_table(cx<T> *tab, const double *k2, double ke_dke, int tid, int nt) {
-    for (int i = tid; i < N; i += nt) tab[i] = expmi<T>((ke_dke * k2[i]));
+    for (int i = tid; i < N; i += nt) tab[i] = cscale(expmi<T>((ke_dke * k2[i])), (T)(1.0 / N));
 }
 
 template <typename T, int N>
@@ -243,7 +245,7 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
             const cx<T> *tab = (const cx<T> *)za.dtab;
             fft_line<T, N, -1, wave_line<T, N>(), false, acc_tw<T, N>()>(v, L, t, tb);
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = cscale(cmul(v[e], tab[t + e * P]), (T)(1.0 / N));
+            for (int e = 0; e < E; ++e) v[e] = cmul(v[e], tab[t + e * P]);
             fft_line<T, N, +1, wave_line<T, N>(), false, acc_tw<T, N>()>(v, L, t, tb);
             if (valid) {
 #pragma unroll
@@ -386,14 +388,25 @@ __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const T
         // spectral indices of this thread's points: along the line i, fixed `other` and k
         if constexpr (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = cscale(cmul(v[e], tab[t + e * P]), (T)(1.0 / N));
+            for (int e = 0; e < E; ++e) v[e] = cmul(v[e], tab[t + e * P]);
         } else {
             const double k2o = k2[other], k2k = valid ? k2[k] : 0.0;
+            if constexpr (sizeof(T) == 4) {
+                // complex floats: the Green factor in FP32 (an approximate
+                // reciprocal, ~1e-7; the potential is recomputed each step)
+                const float g = (float)(sa.green * sa.fftnorm);
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const double kk = k2[t + e * P] + k2o + k2k;
-                const double gf = (kk == 0.0) ? 0.0 : sa.green / kk * sa.fftnorm;
-                v[e] = cscale(v[e], (T)gf);
+                for (int e = 0; e < E; ++e) {
+                    const double kk = k2[t + e * P] + k2o + k2k;
+                    v[e] = cscale(v[e], kk == 0.0 ? 0.f : g * __builtin_amdgcn_rcpf((float)kk));
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const double kk = k2[t + e * P] + k2o + k2k;
+                    const double gf = (kk == 0.0) ? 0.0 : sa.green / kk * sa.fftnorm;
+                    v[e] = cscale(v[e], (T)gf);
+                }
             }
         }
         fft_line<T, N, +1, false, false, ACC>(v, L, t, tw);
