@@ -44,7 +44,7 @@ def pass_bytes(n: int, sc: int = 16, sr: int = 8) -> list[float]:
             n3 * 2 * h * sc, n3 * 2 * h * sc, n3 * 2 * h * sc, n3 * (h * sc + sr), 0.0]
 
 
-def survey_model_bytes(n: int, sc: int = 16) -> float:
+def pencil_model_bytes(n: int, sc: int = 16) -> float:
     """B_alg of the minimal five-pass-drift pencil schedule, (11 + 8h) s_c N^3:
     the reference point of the step's HBM roofline fraction."""
     h = (n // 2 + 1) / n
@@ -384,8 +384,8 @@ def main():
         # frac_of_8TBs: the step on the reference five-pass-drift schedule's
         # bytes, (11 + 8h) s_c N^3 per step at 8 TB/s; bytes_moved_*: this
         # schedule's own algorithmic bytes (fewer: three drift passes)
-        "step_hbm": {"alg_bytes_per_step": survey_model_bytes(n), "frac_of_8TBs":
-                         round(survey_model_bytes(n) * sps / 1e9 / HBM_PEAK_GBS, 4),
+        "step_hbm": {"alg_bytes_per_step": pencil_model_bytes(n), "frac_of_8TBs":
+                         round(pencil_model_bytes(n) * sps / 1e9 / HBM_PEAK_GBS, 4),
                      "bytes_moved_per_step": b_alg, "achieved_GBs": round(step_gbs, 1),
                      "bytes_moved_frac_of_8TBs": round(step_gbs / HBM_PEAK_GBS, 4),
                      "cell_updates_per_s": round(sps * n ** 3, 1),
@@ -403,7 +403,7 @@ def main():
             "value": round(spsm, 3), "unit": "KDK steps/s", "ms_per_step": round(1e3 * elm / args.steps, 4),
             "speedup_vs_fp64": round(spsm / sps, 3), "dtype": "c64 storage/FP32 FFT, FP64 phases+sums",
             "roofline": roofline(n, avgm, 8, 4, "mixed"),
-            "step_hbm_frac_of_8TBs": round(survey_model_bytes(n, 8) * spsm / 1e9 / HBM_PEAK_GBS, 4),
+            "step_hbm_frac_of_8TBs": round(pencil_model_bytes(n, 8) * spsm / 1e9 / HBM_PEAK_GBS, 4),
             "bytes_moved_frac_of_8TBs": round(sum(pbm) * spsm / 1e9 / HBM_PEAK_GBS, 4),
             "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avgm)},
         }

@@ -1,6 +1,6 @@
 """GPU parity: libifdm.so (HIP, gfx950) against the CPU oracle on the same
 seeded inputs.  FP64 bar: relative L2 <= 1e-10 on psi / phi after the steps
-(the stated north-star tolerance), 1e-12 on single transforms, and norm
+(the stated FP64 tolerance), 1e-12 on single transforms, and norm
 sum |psi|^2 conserved to 1e-12."""
 import math
 import sys
