@@ -73,9 +73,16 @@ template <int N, int E> constexpr int ns_before(int s) {
 
 // LDS position of point p of a line: one pad slot per 8 points.
 __device__ __forceinline__ int lpos(int p) { return p + (p >> 3); }
-// LDS line stride in elements: congruent to 2 mod 16 so that lines of a
-// column tile fall on different banks.
+// LDS line stride in elements of the z passes: congruent to 2 mod 16 so that
+// the lines of a wavefront holding several short lines fall on different
+// banks.
 template <int N> constexpr int lds_line() { return ((N + N / 8 + 15) / 16) * 16 + 2; }
+// Line stride of the strided passes' column tiles, whose wavefront lanes run
+// over the columns first: congruent to 9 mod 16.  Measured at 512^3 (LDS
+// bank-conflict cycles per conflict-free LDS cycle): 1.0 (FP64) and 0.67
+// (complex floats) at 2 mod 16, 0.67 and 0.33 at 1 mod 16, 0.33 in both
+// at 9 mod 16.
+template <int N> constexpr int lds_col_line() { return ((N + N / 8 + 15) / 16) * 16 + 9; }
 
 // ---------------------------------------------------------------- butterflies
 template <int DIR, typename T> __device__ __forceinline__ void bfly2(cx<T> &a0, cx<T> &a1) {

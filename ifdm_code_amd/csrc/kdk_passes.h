@@ -109,7 +109,7 @@ template <typename T, int N, bool WIDE = false> constexpr int s_cols() {
     constexpr int want = (WIDE && sizeof(cx<T>) == 16) ? 2 * base : base;
     constexpr int cap = 1024 / (N / SchedT<T, N>::E);
     // and the exchange buffer within the 160 KB of LDS
-    constexpr int lcap = 163840 / (lds_line<N>() * (int)sizeof(cx<T>));
+    constexpr int lcap = 163840 / (lds_col_line<N>() * (int)sizeof(cx<T>));
     int c = want < cap ? want : cap;
     while (c > lcap) c /= 2;
     return c;
@@ -466,7 +466,7 @@ template <typename T, int MODE> constexpr int sp_wpe() { return (MODE == S_GREEN
 template <typename T, int N, int AXIS, int MODE, bool WIDE>
 __global__ __launch_bounds__((s_cols<T, N, WIDE>() * (N / SchedT<T, N>::E)), (sp_wpe<T, MODE>()))
 void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa) {
-    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE>(), LS = lds_line<N>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE>(), LS = lds_col_line<N>();
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     int bx = blockIdx.x, by = blockIdx.y;
@@ -510,7 +510,7 @@ template <typename T, int N, int AXIS, int MODE>
 __global__ __launch_bounds__((s_cols<T, N>() * (N / SchedT<T, N>::E)))
 void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa, int ntz,
                 int ntiles) {
-    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N>(), LS = lds_line<N>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N>(), LS = lds_col_line<N>();
     constexpr int NT = TC * P;
     __shared__ cx<T> lds[TC * LS];
     // the spectral multiply's table in LDS, so no global load is issued
