@@ -151,24 +151,28 @@ def round_c64(psi: np.ndarray) -> np.ndarray:
     return (psi * f).astype(np.complex64).astype(np.complex128) / f
 
 
-def split_schedule(F, cfg, psi, total: int, a_stop: float = 0.5):
+def split_schedule(F, cfg, psi, total: int, a_stop: float = 0.5, mode: str = "split"):
     """The FP64 adaptive steps (Timestep_criteria each step) up to the
     reference's own end of run, the first step that takes a past a_stop
-    (src/CosmoBEC_MPI_stan.cpp:250-255); when `total` asks for more steps
-    than that, each is split into m equal parts (D_KE, D_PE, D_QI divided by
-    m) so the same expansion span is covered.  Returns (the first `total`
-    steps, adaptive step count, a at the end of the adaptive run, m)."""
+    (src/CosmoBEC_MPI_stan.cpp:250-255).  When `total` asks for more steps
+    than that: mode "split" divides each into m equal parts (D_KE, D_PE,
+    D_QI divided by m) so the same expansion span is covered; mode "hold"
+    repeats the last step's integrals until `total` (the run then continues
+    past a_stop).  Returns (the first `total` steps, adaptive step count, a at
+    the end of the adaptive run, m; m = 1 for "hold")."""
     sched, a = [], 1.0 / 51
     with F.Solver(cfg) as sg:
         sg.set_psi(psi)
         while a <= a_stop and len(sched) < total:
             a, d = sg.simulate_native(a, 1)
             sched += d
+    if mode == "hold":
+        return (sched + [sched[-1]] * (total - len(sched)))[:total], len(sched), a, 1
     m = max(1, -(-total // len(sched)))
     return [tuple(x / m for x in d) for d in sched for _ in range(m)][:total], len(sched), a, m
 
 
-def error_study(F, cfg, cfgm, psi, checkpoints, a_stop: float = 0.5) -> dict:
+def error_study(F, cfg, cfgm, psi, checkpoints, a_stop: float = 0.5, mode: str = "split") -> dict:
     """Precision error against the FP64 GPU run of the same IC.
 
     Schedule: the FP64 adaptive run (Timestep_criteria each step) up to the
@@ -176,14 +180,15 @@ def error_study(F, cfg, cfgm, psi, checkpoints, a_stop: float = 0.5) -> dict:
     (src/CosmoBEC_MPI_stan.cpp:250-255).  When the largest checkpoint asks
     for more steps than that, every step is split into m equal parts
     (D_KE, D_PE, D_QI each divided by m) so the same expansion span is
-    covered in that many steps.  The FP64 path, the mixed path and an FP64
+    covered in that many steps (mode "split"; mode "hold" repeats the last
+    adaptive step instead, past a = 0.5).  The FP64 path, the mixed path and an FP64
     run started from the IC rounded to complex64 (the error the rounding of
     the IC alone grows into: the floor the dynamics set) replay that
     schedule; at each checkpoint (cumulative step counts) the per-field
     errors of the latter two against the FP64 path are recorded."""
-    steps, nad, a, m = split_schedule(F, cfg, psi, max(checkpoints), a_stop)
+    steps, nad, a, m = split_schedule(F, cfg, psi, max(checkpoints), a_stop, mode)
     cps = sorted({min(c, len(steps)) for c in checkpoints})
-    log(f"[bench] error study: {nad} adaptive steps to a = {a:.4f}, split {m}x -> {len(steps)} steps")
+    log(f"[bench] error study: {nad} adaptive steps to a = {a:.4f}, {mode} (m = {m}) -> {len(steps)} steps")
     s64, smx, sfl = F.Solver(cfg), F.Solver(cfgm), F.Solver(cfg)
     try:
         for s, p in ((s64, psi), (smx, psi), (sfl, round_c64(psi))):
@@ -203,7 +208,8 @@ def error_study(F, cfg, cfgm, psi, checkpoints, a_stop: float = 0.5) -> dict:
             log(f"[bench] error after {c} steps: {json.dumps(row)}")
             rows.append(row)
             del psi64, phi64
-        return {"adaptive_steps": nad, "a_end": a, "split": m, "steps": len(steps), "checkpoints": rows}
+        return {"adaptive_steps": nad, "a_end": a, "schedule": mode, "split": m, "steps": len(steps),
+                "checkpoints": rows}
     finally:
         for s in (s64, smx, sfl):
             s.close()
@@ -322,6 +328,8 @@ def main():
     ap.add_argument("--error-steps", type=str, default="100,1000",
                     help="comma list of cumulative step counts at which the mixed path's error against the "
                          "FP64 run is reported ('' to skip)")
+    ap.add_argument("--error-schedule", choices=("split", "hold"), default="split",
+                    help="past the adaptive run's end at a = 0.5: split every step (default) or hold the last one")
     ap.add_argument("--no-mixed-check", action="store_true",
                     help="skip the mixed path's replay of the timed FP64 schedule (host memory at 1024^3)")
     ap.add_argument("--adjoint-steps", type=int, default=0,
@@ -437,7 +445,7 @@ def main():
         del psi64, rho64, w, phi64
     cps = [int(x) for x in args.error_steps.split(",") if x.strip()]
     if not args.no_mixed and cps:
-        out["mixed"]["error_vs_fp64"] = error_study(F, cfg, cfgm, psi, sorted(cps))
+        out["mixed"]["error_vs_fp64"] = error_study(F, cfg, cfgm, psi, sorted(cps), mode=args.error_schedule)
     if args.adjoint_steps:
         out["adjoint"] = adjoint_study(F, n, xmax_in, g_in, args.adjoint_steps, args.adjoint_k, args.adjoint_fd)
     if not args.no_cpu_baseline:
