@@ -66,13 +66,15 @@ def make_ic(F, n: int, xmax_in: float, seed: int = 1234) -> np.ndarray:
         return s.get_psi()
 
 
-def cpu_baseline(n: int, psi: np.ndarray, cfg_g_in: float, xmax_in: float, steps: int = 2) -> dict:
+def cpu_baseline(n: int, psi: np.ndarray, cfg_g_in: float, xmax_in: float, steps: int = 5, warm: int = 2) -> dict:
+    """The oracle's C restatement of the reference step on the host's cores:
+    `warm` untimed steps, then the median of `steps` timed ones."""
     from oracle import oracle as O
     rp = O.RunParams(n=n, mfdm=2.5e-24, g_in=cfg_g_in, xmax_in=xmax_in)
     st = O.State(rp, psi)
     a = 1.0 / 51
     times = []
-    for i in range(steps + 1):
+    for i in range(warm + steps):
         t0 = time.perf_counter()
         st.poisson()
         m = st.maxima()
@@ -80,10 +82,11 @@ def cpu_baseline(n: int, psi: np.ndarray, cfg_g_in: float, xmax_in: float, steps
         st.split_op(dKE, dPE, dQI)
         times.append(time.perf_counter() - t0)
         log(f"[cpu baseline] step {i} {times[-1]:.2f} s")
-    t = float(np.median(times[1:]))
+    t = float(np.median(times[warm:]))
     return {"value": 1.0 / t, "unit": "KDK steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"{steps} timed steps (+1 warm-up) of the same {n}^3 FP64 workload with the oracle's C "
-                      f"restatement (poisson + Timestep_criteria + Split_op_MPI, 6 3-D FFTs/step), OpenMP"}
+            "sample": f"median of {steps} timed steps after {warm} warm-up steps of the same {n}^3 FP64 workload "
+                      f"with the oracle's C restatement (poisson + Timestep_criteria + Split_op_MPI, 6 3-D FFTs "
+                      f"per step), OpenMP", "step_s": [round(x, 3) for x in times[warm:]]}
 
 
 def rel_l2(a, b) -> float:
@@ -323,7 +326,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--prof-steps", type=int, default=5)
     ap.add_argument("--error-steps", type=str, default="100,1000",
                     help="comma list of cumulative step counts at which the mixed path's error against the "
