@@ -109,7 +109,7 @@ struct Ops {
           constexpr int bit = PSI ? (AXIS == 1 ? 1 : 2) : (AXIS == 0 ? 8 : (MODE == S_INV ? 16 : 4));
           if (c->persistent & bit) {
             done = true;
-            constexpr int TC = s_cols<T, N>();
+            constexpr int TC = s_cols<T, N, false, MODE>();
             const int ntz = (kv + TC - 1) / TC;
             static int occ = -1;   // resident work-groups per CU for this instantiation
             if (occ < 0) {
@@ -126,7 +126,7 @@ struct Ops {
           }
         }
         if (!done) {
-            constexpr int TC = s_cols<T, N, WIDE>();
+            constexpr int TC = s_cols<T, N, WIDE, MODE>();
             const int ntz = (kv + TC - 1) / TC;
             hipLaunchKernelGGL((k_spass<T, N, AXIS, MODE, WIDE>), dim3(ntz, no), dim3(TC * P), 0, c->stream,
                                (cx<T> *)arr, rl, ps, kv, (const cx<T> *)c->tw, (const cx<double> *)c->tw64, sa);

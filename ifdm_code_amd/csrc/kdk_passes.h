@@ -103,9 +103,13 @@ template <typename T, int N> constexpr int z_lines() {
 }
 // z-columns per strided-pass tile: 128-byte row segments (8 complex
 // doubles, 16 complex floats); WIDE doubles them for FP64 (256-byte rows,
-// used on the x pass over psi), and a work-group never exceeds 1024 threads
-template <typename T, int N, bool WIDE = false> constexpr int s_cols() {
-    constexpr int base = 128 / (int)sizeof(cx<T>);
+// used on the x pass over psi), and a work-group never exceeds 1024 threads.
+// Complex floats at N = 1024 take 64-byte rows on the half spectrum's y
+// passes F and H, so two work-groups fit a CU's LDS (measured at 1024^3:
+// F 2.50 -> 1.95 ms, H 2.32 -> 1.90 ms; passes A, B and G lose with them).
+template <typename T, int N, bool WIDE = false, int MODE = -1> constexpr int s_cols() {
+    constexpr bool narrow = sizeof(T) == 4 && N == 1024 && !WIDE && (MODE == S_FWD || MODE == S_INV);
+    constexpr int base = narrow ? 8 : 128 / (int)sizeof(cx<T>);
     constexpr int want = (WIDE && sizeof(cx<T>) == 16) ? 2 * base : base;
     constexpr int cap = 1024 / (N / SchedT<T, N>::E);
     // and the exchange buffer within the 160 KB of LDS
@@ -464,9 +468,9 @@ template <typename T, int MODE> constexpr int sp_wpe() { return (MODE == S_GREEN
 
 // one tile per work-group: grid (z-column tiles, lines of the fixed axis)
 template <typename T, int N, int AXIS, int MODE, bool WIDE>
-__global__ __launch_bounds__((s_cols<T, N, WIDE>() * (N / SchedT<T, N>::E)), (sp_wpe<T, MODE>()))
+__global__ __launch_bounds__((s_cols<T, N, WIDE, MODE>() * (N / SchedT<T, N>::E)), (sp_wpe<T, MODE>()))
 void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa) {
-    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE>(), LS = lds_col_line<N>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE, MODE>(), LS = lds_col_line<N>();
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     int bx = blockIdx.x, by = blockIdx.y;
@@ -507,10 +511,10 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
 // overlap the butterflies and LDS exchanges.  The first LDS write of every
 // transform is preceded by a barrier, so consecutive tiles never race on LDS.
 template <typename T, int N, int AXIS, int MODE>
-__global__ __launch_bounds__((s_cols<T, N>() * (N / SchedT<T, N>::E)))
+__global__ __launch_bounds__((s_cols<T, N, false, MODE>() * (N / SchedT<T, N>::E)))
 void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa, int ntz,
                 int ntiles) {
-    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N>(), LS = lds_col_line<N>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, false, MODE>(), LS = lds_col_line<N>();
     constexpr int NT = TC * P;
     __shared__ cx<T> lds[TC * LS];
     // the spectral multiply's table in LDS, so no global load is issued

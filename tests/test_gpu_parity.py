@@ -435,3 +435,54 @@ def test_plane_wave_drift_1024(precision):
         err = max(err, float(np.max(np.abs(out[i:i + 128] / psi[i:i + 128] - want))))
     _say(f"   max |psi'/psi - exp(-i 0.3)| = {err:.3e}")
     assert err < (1e-11 if precision == 0 else 2e-5)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_poisson_cosine_1024(precision):
+    """The Poisson solve (source, the half-spectrum passes, c2r) at 1024^3:
+    a density with one Fourier mode, |psi|^2 = A^2 (1 + 0.1 cos k.x), has
+    the potential C cos k.x for the same physical k at any mesh size
+    (-4 pi G / |k|^2 on the continuous k, src/CosmoBEC_MPI_stan.cpp:104-110;
+    the constant part of the source is the k = 0 mode, set to zero).  C is
+    taken from the FP64 solve at 32^3 (pinned against the oracle by
+    test_poisson_matches_oracle); the 1024^3 potential must be C cos k.x."""
+    import math
+    F, O = _mods()
+    m = (3, -5, 7)
+
+    def field(n, rp):
+        kv = O.kvec(n, rp.xmax)
+        x = rp.xmax / n - rp.xmax + np.arange(n) * rp.dx
+        return [np.exp(1j * kv[mi % n] * (x + rp.xmax)) for mi in m]
+
+    def solve(n, prec, chunk):
+        rp = _rp(O, n)
+        cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n, precision=prec)
+        amp = math.sqrt(rp.rhoc / rp.mfdm_solar)
+        ex, ey, ez = field(n, rp)
+        psi = np.empty((n, n, n), np.complex128)
+        for i in range(0, n, chunk):
+            c = (ex[i:i + chunk, None, None] * ey[None, :, None] * ez[None, None, :]).real
+            psi[i:i + chunk] = amp * np.sqrt(1.0 + 0.1 * c)
+        with F.Solver(cfg) as s:
+            s.set_psi(psi)
+            del psi
+            st = s.poisson()
+            return s.get_phi(), st, (ex, ey, ez)
+
+    phi32, _st, (ex, ey, ez) = solve(32, 0, 32)
+    c32 = (ex[:, None, None] * ey[None, :, None] * ez[None, None, :]).real
+    C = float(np.sum(phi32 * c32) / np.sum(c32 * c32))
+    assert rel_l2(phi32, C * c32) < 1e-13              # one mode at 32^3 already
+    n = 1024
+    _say(f"[poisson cosine] n={n} precision={precision} C={C:.6e}")
+    phi, st, (ex, ey, ez) = solve(n, precision, 64)
+    err = 0.0
+    for i in range(0, n, 64):                          # chunks keep the temporaries small
+        c = (ex[i:i + 64, None, None] * ey[None, :, None] * ez[None, None, :]).real
+        err = max(err, float(np.max(np.abs(phi[i:i + 64] - C * c))))
+    _say(f"   max |phi - C cos k.x| / |C| = {err / abs(C):.3e}, max_phi / |C| = {st['max_phi'] / abs(C):.12f}")
+    assert err / abs(C) < (1e-11 if precision == 0 else 1e-5)
+    # the lattice phases are 2 pi (3i - 5j + 7k + 2.5) / n: the nearest to a
+    # multiple of 2 pi is pi / n away
+    assert st["max_phi"] == pytest.approx(abs(C) * math.cos(math.pi / n), rel=1e-9 if precision == 0 else 1e-5)
