@@ -394,20 +394,20 @@ double kvec(int i, int n, double xmax) {
 
 void release(ifdm_ctx *c) {
     if (!c) return;
-    hipSetDevice(c->cfg.device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &e : c->ev_pool)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     void *icb[] = {c->ic_rho, c->ic_vx, c->ic_vy, c->ic_vz, c->ic_skipped};
     for (void *b : icb)
-        if (b) hipFree(b);
+        if (b) (void)hipFree(b);
     void *bufs[] = {c->psi, c->phi, c->shat, c->tw, c->k2, c->part_rho, c->part_phi, c->red, c->red_stage, c->dtab,
                     c->adj_g, c->adj_s, c->adj_phs, c->ck};
     for (void *b : bufs)
-        if (b) hipFree(b);
-    if (c->tw64 && c->tw64 != c->tw) hipFree(c->tw64);
-    if (c->red_host) hipHostFree(c->red_host);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (b) (void)hipFree(b);
+    if (c->tw64 && c->tw64 != c->tw) (void)hipFree(c->tw64);
+    if (c->red_host) (void)hipHostFree(c->red_host);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -779,7 +779,7 @@ int ifdm_internal_theta_solve(ifdm_ctx *c, void *buf64, double coef) {
     if (!rc) rc = d.theta_solve(&sh, coef);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = IFDM_E_HIP;
     if (rc && c->err.empty()) c->err = sh.err.empty() ? "theta solve failed" : sh.err;
-    hipFree(sh.tw);
+    (void)hipFree(sh.tw);
     sh.stream = nullptr;
     return rc;
 }
@@ -800,7 +800,7 @@ int ifdm_power_spectrum(ifdm_ctx *c, int nbins, double *k_mean, double *pk, doub
         c->err = "power spectrum copy failed";
         rc = IFDM_E_HIP;
     }
-    hipFree(dev);
+    (void)hipFree(dev);
     c->ev_pass.clear();
     if (rc) return rc;
     // P(k) = L^3 |delta_k|^2 / N^6 with delta = (m_s|psi|^2 - rho_mean) / rho_mean
@@ -824,7 +824,7 @@ int ifdm_run_checkpointed(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, int K) 
     if ((rc = ifdm_internal_flush(c))) return rc;
     const size_t bytes = (c->mixed ? 2 * sizeof(float) : 2 * sizeof(double)) * (size_t)c->np;
     if (c->ck) {
-        hipFree(c->ck);
+        (void)hipFree(c->ck);
         c->ck = nullptr;
     }
     c->ck_nsteps = -1;
@@ -984,32 +984,34 @@ int ifdm_stream_copy(ifdm_ctx *c, size_t bytes, int iters, double *gbs) {
     void *a = nullptr, *b = nullptr;
     HIPCHK(c, hipMalloc(&a, n * 16));
     if (hipMalloc(&b, n * 16) != hipSuccess) {
-        hipFree(a);
+        (void)hipFree(a);
         c->err = "hipMalloc failed in ifdm_stream_copy";
         return IFDM_E_NOMEM;
     }
-    hipEvent_t e0, e1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = IFDM_OK;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipMemsetAsync(a, 0, n * 16, c->stream);
-    const unsigned nb = (unsigned)std::min<long>((n + 255) / 256, 1L << 20);
-    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, c->stream, (const double2 *)a, (double2 *)b, n);  // warm
-    hipEventRecord(e0, c->stream);
-    for (int i = 0; i < iters; ++i)
-        hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, c->stream, (const double2 *)a, (double2 *)b, n);
-    hipEventRecord(e1, c->stream);
     float ms = 0.f;
-    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess ||
-        hipGetLastError() != hipSuccess) {
-        c->err = "copy kernel failed";
+    const unsigned nb = (unsigned)std::min<long>((n + 255) / 256, 1L << 20);
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        hipMemsetAsync(a, 0, n * 16, c->stream) != hipSuccess) {
+        c->err = "event or memset failed in ifdm_stream_copy";
         rc = IFDM_E_HIP;
+    } else {
+        hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, c->stream, (const double2 *)a, (double2 *)b, n);  // warm
+        if (hipEventRecord(e0, c->stream) != hipSuccess) rc = IFDM_E_HIP;
+        for (int i = 0; i < iters && !rc; ++i)
+            hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, c->stream, (const double2 *)a, (double2 *)b, n);
+        if (!rc && (hipGetLastError() != hipSuccess || hipEventRecord(e1, c->stream) != hipSuccess ||
+                    hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = IFDM_E_HIP;
+        if (rc) c->err = "copy kernel failed";
     }
-    *gbs = 2.0 * 16.0 * (double)n * iters / (ms * 1e-3) / 1e9;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipFree(a);
-    hipFree(b);
+    if (!rc) *gbs = 2.0 * 16.0 * (double)n * iters / (ms * 1e-3) / 1e9;
+    // release whatever was created (errors here leave nothing to undo)
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return rc;
 }
 
@@ -1068,7 +1070,7 @@ int ifdm_time_passes(ifdm_ctx *c, const ifdm_dt *dt, int iters, float *ms_out) {
             acc[p] += ms;
         }
     }
-    for (auto &e : ev) hipEventDestroy(e);
+    for (auto &e : ev) (void)hipEventDestroy(e);
     if (rc) { if (c->err.empty()) c->err = "event timing failed"; return rc; }
     for (int p = 0; p < IFDM_NPASS; ++p) ms_out[p] = (float)(acc[p] / iters);
     return IFDM_OK;

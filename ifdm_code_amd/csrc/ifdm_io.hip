@@ -26,7 +26,7 @@ unsigned grid_for(long n) { return (unsigned)std::min<long>((n + 255) / 256, 1L 
 void free_ic(ifdm_ctx *c) {
     void *b[] = {c->ic_rho, c->ic_vx, c->ic_vy, c->ic_vz, c->ic_skipped};
     for (void *p : b)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     c->ic_rho = c->ic_vx = c->ic_vy = c->ic_vz = nullptr;
     c->ic_skipped = nullptr;
 }
@@ -108,7 +108,7 @@ int ifdm_ic_deposit(ifdm_ctx *c, const float *pos, const float *vel, long long n
     const long long cap = std::min<long long>(std::max<long long>(np, 1), batch);
     HIPCHK(c, hipMalloc((void **)&dp, sizeof(float) * 3 * cap));
     if (hipMalloc((void **)&dv, sizeof(float) * 3 * cap) != hipSuccess) {
-        hipFree(dp);
+        (void)hipFree(dp);
         c->err = "hipMalloc failed for a particle batch";
         return IFDM_E_NOMEM;
     }
@@ -132,8 +132,8 @@ int ifdm_ic_deposit(ifdm_ctx *c, const float *pos, const float *vel, long long n
     };
     if (ghost0 < np) deposit(ghost0, np);
     deposit(0, np);
-    hipFree(dp);
-    hipFree(dv);
+    (void)hipFree(dp);
+    (void)hipFree(dv);
     if (!rc) c->ic_npart += np + (np - ghost0);
     return rc;
 }
@@ -189,7 +189,7 @@ int ifdm_ic_finish(ifdm_ctx *c, double a, double *theta_out, float *vx_out, long
         if (hipMemcpy(&sk, c->ic_skipped, sizeof(sk), hipMemcpyDeviceToHost) != hipSuccess) rc = IFDM_E_HIP;
         *skipped_out = (long long)sk;
     }
-    if (c->mixed && buf) hipFree(buf);
+    if (c->mixed && buf) (void)hipFree(buf);
     free_ic(c);
     if (rc && c->err.empty()) c->err = "initial-condition assembly failed";
     return rc;

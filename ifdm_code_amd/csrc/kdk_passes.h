@@ -462,13 +462,16 @@ __device__ __forceinline__ void xcd_pair(int &bx, int &by) {
 
 // Minimum waves per SIMD asked of the compiler for a strided pass: 8 caps
 // the complex-float Green pass at 64 VGPRs, so two 1024-thread work-groups
-// share a CU (measured at 512^3: 0.39 -> 0.35 ms, one 8-byte spill); the
-// other passes fit without a cap (capping the z-pass C spilled and slowed).
-template <typename T, int MODE> constexpr int sp_wpe() { return (MODE == S_GREEN && sizeof(T) == 4) ? 8 : 1; }
+// share a CU (measured at 512^3: 0.39 -> 0.35 ms, one 8-byte spill), for
+// the sizes where two such work-groups fit the LDS; the other passes fit
+// without a cap (capping the z-pass C spilled and slowed).
+template <typename T, int MODE, int N> constexpr int sp_wpe() {
+    return (MODE == S_GREEN && sizeof(T) == 4 && N >= 128 && N <= 512) ? 8 : 1;
+}
 
 // one tile per work-group: grid (z-column tiles, lines of the fixed axis)
 template <typename T, int N, int AXIS, int MODE, bool WIDE>
-__global__ __launch_bounds__((s_cols<T, N, WIDE, MODE>() * (N / SchedT<T, N>::E)), (sp_wpe<T, MODE>()))
+__global__ __launch_bounds__((s_cols<T, N, WIDE, MODE>() * (N / SchedT<T, N>::E)), (sp_wpe<T, MODE, N>()))
 void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa) {
     constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE, MODE>(), LS = lds_col_line<N>();
     __shared__ cx<T> lds[TC * LS];
@@ -593,8 +596,7 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__r
         }
         __builtin_amdgcn_sched_barrier(0);   // take over the prefetch only after the stores
         if (!more) break;
-#pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = nx[e];
+        for (int e = 0; e < E; ++e) v[e] = nx[e];   // register moves (becomes an array copy: no unroll pragma)
         tile = nt;
     }
 }
