@@ -468,7 +468,7 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     // measured neutral)
     c->sps = (long)cfg->n * c->hs;
     // persistent work-groups measured faster on the FP64 passes G and H only
-    c->persistent = c->mixed ? 0 : 24;
+    c->persistent = c->mixed ? 0 : 8;
     const int n = c->n;
     auto fail = [&](int code, const std::string &m) {
         g_create_err = m;

@@ -453,12 +453,15 @@ __device__ __forceinline__ void spass_addr(int other, int k, int rl, long ps, lo
 // registers) plus a per-lane 32-bit byte offset, which maps to the
 // scalar-base + vector-offset form of global loads and stores.  Up to
 // N = 512 every offset inside an array fits 32 bits; N = 1024 uses 64-bit
-// addresses.  Stores are non-temporal; NTL makes the loads non-temporal
-// too (each tile is read once per pass).  Measured at 512^3 on top of the
-// non-temporal stores: FP64 202.4 -> 205.9 steps/s, mixed 370.4 -> 384;
-// every strided pass gains except the 256-byte tiles of the FP64 x pass
-// (0.93 -> 0.99 ms), which keep plain loads.
-template <typename T, int N, bool NTL = false> struct TileIO {
+// addresses.  NTS, NTL: non-temporal stores, loads (each tile is read and
+// written once per pass).  Measured at 512^3, loads on top of the stores:
+// FP64 202.4 -> 205.9 steps/s, mixed 370.4 -> 384; every strided pass
+// gains except the 256-byte tiles of the FP64 x pass (0.93 -> 0.99 ms),
+// which keep plain loads.  Tiles with rows narrower than a 128-byte line
+// (complex floats on F and H at N = 1024) keep plain accesses: two
+// work-groups share each line there, and the second finds it in L2 only
+// if the first kept it (non-temporal: F 1.95 -> 3.0 ms at 1024^3).
+template <typename T, int N, bool NTL = false, bool NTS = true> struct TileIO {
     static constexpr bool k32 = N <= 512;
     const char *base;   // wave-uniform
     __device__ __forceinline__ TileIO(const cx<T> *a, long uniform_elem) {
@@ -477,8 +480,11 @@ template <typename T, int N, bool NTL = false> struct TileIO {
         }
     }
     __device__ __forceinline__ void store(long elem, cx<T> v) const {
-        if constexpr (k32) stream_store(reinterpret_cast<cx<T> *>(const_cast<char *>(base) + (unsigned)((unsigned)elem * sizeof(cx<T>))), v);
-        else stream_store(reinterpret_cast<cx<T> *>(const_cast<char *>(base)) + elem, v);
+        cx<T> *p;
+        if constexpr (k32) p = reinterpret_cast<cx<T> *>(const_cast<char *>(base) + (unsigned)((unsigned)elem * sizeof(cx<T>)));
+        else p = reinterpret_cast<cx<T> *>(const_cast<char *>(base)) + elem;
+        if constexpr (NTS) stream_store(p, v);
+        else *p = v;
     }
 };
 
@@ -519,7 +525,9 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
     const bool valid = k < kv;
     long base0, stride;
     spass_addr<N, AXIS>(other, k0, rl, ps, base0, stride);
-    const TileIO<T, N, !(WIDE && sizeof(T) == 8)> io(a, base0);   // plain loads on 256-byte tiles
+    constexpr bool ROW128 = TC * sizeof(cx<T>) >= 128;
+    // plain loads on the FP64 256-byte tiles and on rows narrower than a line
+    const TileIO<T, N, ROW128 && !(WIDE && sizeof(T) == 8), ROW128> io(a, base0);
     cx<T> *L = lds + c * LS;
     constexpr bool PSI = MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1;
     TwBase<T, N, tw_base_t<T, N, PSI>> tb;
