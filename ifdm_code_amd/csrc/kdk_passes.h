@@ -103,7 +103,8 @@ template <typename T, int N> constexpr int z_lines() {
 }
 // z-columns per strided-pass tile: 128-byte row segments (8 complex
 // doubles, 16 complex floats); WIDE doubles them for FP64 (256-byte rows,
-// used on the x pass over psi), and a work-group never exceeds 1024 threads.
+// used by the 3-D transform's x pass), and a work-group never exceeds 1024
+// threads.
 // Complex floats at N = 1024 take 64-byte rows on the half spectrum's y
 // passes F and H, so two work-groups fit a CU's LDS (measured at 1024^3:
 // F 2.50 -> 1.95 ms, H 2.32 -> 1.90 ms; passes A, B and G lose with them).
