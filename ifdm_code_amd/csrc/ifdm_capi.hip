@@ -456,13 +456,17 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     c->n3 = (long)cfg->n * cfg->n * cfg->n;
     c->hs = hs_of(cfg->n);
     c->mixed = cfg->precision == IFDM_MIXED;
-    // x-planes of psi and phi start 32 cells apart beyond their N*N cells: a
+    // x-planes of psi and phi are padded beyond their N*N cells: a
     // power-of-two plane stride sends the rows of an x-line to the same HBM
-    // channels (measured at 512^3: the x pass B 1.09 -> 0.96 ms in FP64,
+    // channels (measured at 512^3 with 32 cells: the x pass B 1.09 -> 0.96 ms in FP64,
     // 0.59 -> 0.44 ms mixed; 15.4 -> 10.5 ms FP64 at 1024^3).  Pads round up
     // to 8 cells, so rows stay 8-cell aligned, as the phi lines that pass I
-    // stores as complex pairs need.
-    long pad = cfg->plane_pad > 0 ? cfg->plane_pad : (cfg->plane_pad < 0 ? 0 : 32);
+    // stores as complex pairs need.  Built-in: 256 bytes of psi, 16 cells in
+    // FP64 and 32 in mixed (a sweep of 16, 32 and 64 cells at 512^3 with
+    // the non-temporal accesses: FP64 213.5, 211.7, 208.9 steps/s, pass B
+    // 0.86, 0.90, 0.96 ms; mixed within 0.5 % across the three).
+    const long dpad = c->mixed ? 32 : 16;
+    long pad = cfg->plane_pad > 0 ? cfg->plane_pad : (cfg->plane_pad < 0 ? 0 : dpad);
     pad = (pad + 7) / 8 * 8;
     c->ps = (long)cfg->n * cfg->n + pad;
     c->np = (long)cfg->n * c->ps;

@@ -56,7 +56,8 @@ typedef struct {
     int device;            /* HIP device ordinal */
     int plane_pad;         /* device layout of psi and phi: cells of padding
                               after each x-plane of N*N cells, rounded up to
-                              a multiple of 8 (0: the built-in 32 cells;
+                              a multiple of 8 (0: the built-in 256 bytes,
+                              16 cells in FP64 and 32 in mixed;
                               negative: none; at most 65536).  Host arrays
                               stay dense. */
 } ifdm_config;
