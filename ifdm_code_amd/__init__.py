@@ -275,9 +275,14 @@ class Solver:
         m = len(schedule)
         return (_Dt * max(m, 1))(*[_Dt(*x) for x in schedule]), m
 
-    def run_checkpointed(self, schedule, K: int):
+    def run_checkpointed(self, schedule, K: int) -> int:
+        """Forward run from the current psi (phi re-solved on entry) keeping
+        checkpoints for one reverse sweep.  Returns a serial number that
+        changes with every such run (the autograd binding checks it)."""
         arr, m = self._sched(schedule)
         self._chk(lib().ifdm_run_checkpointed(self._h, arr, m, int(K)), "ifdm_run_checkpointed")
+        self.ck_serial = getattr(self, "ck_serial", 0) + 1
+        return self.ck_serial
 
     def adjoint(self, schedule, g_T: np.ndarray) -> np.ndarray:
         """Gradient w.r.t. psi_0 from the gradient w.r.t. psi_T (both in the

@@ -24,7 +24,9 @@ class KDKEvolve(torch.autograd.Function):
         src = psi0.detach().contiguous()
         torch.cuda.synchronize(src.device)
         solver.set_psi_device(src.data_ptr())
-        solver.run_checkpointed(schedule, int(K))
+        # run_checkpointed solves Poisson for psi0 first, so the result does
+        # not depend on what the solver held before
+        ctx.serial = solver.run_checkpointed(schedule, int(K))
         out = torch.empty_like(src)
         solver.get_psi_device(out.data_ptr())
         ctx.solver = solver
@@ -33,9 +35,15 @@ class KDKEvolve(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g):
+        # the solver keeps the checkpoints of its last forward run only, and
+        # they serve one reverse sweep
+        if getattr(ctx.solver, "ck_serial", None) != ctx.serial:
+            raise RuntimeError("evolve(): the solver ran another forward pass after this one; "
+                               "its checkpoints are gone (use one solver per graph)")
         g = g.detach().contiguous().clone()
         torch.cuda.synchronize(g.device)
         ctx.solver.adjoint_device(ctx.schedule, g.data_ptr())
+        ctx.solver.ck_serial = None
         return g, None, None, None
 
 

@@ -58,6 +58,7 @@ struct ifdm_ctx {
     void *adj_g = nullptr, *adj_s = nullptr, *adj_phs = nullptr;
     void *ck = nullptr;
     int ck_k = 0, ck_count = 0, ck_nsteps = -1;
+    std::vector<ifdm_dt> ck_sched;   // the schedule the checkpoints were taken on
     // strided passes with persistent prefetching work-groups: bits 0, 1 the
     // y, x passes on psi; bits 2, 3, 4 the passes F, G, H on the half spectrum
     int persistent = 0;

@@ -1,5 +1,5 @@
 """Computes the 21-point Gauss-Kronrod rule on [-1, 1] in 60-digit precision
-and prints it as the C++ table used by quadrature.cpp.
+and prints it as the C++ table of the qk21 rule in cosmo.cpp.
 
 Gauss nodes: roots of the Legendre polynomial P_10.  Kronrod nodes: roots of
 the Stieltjes polynomial E_11, the monic odd polynomial of degree 11 with

@@ -134,6 +134,23 @@ __global__ __launch_bounds__(256) void k_store_scaled(const cx<double> *__restri
     }
 }
 
+// partial sums of |z|^2 (FP64) of a dense complex128 array, one per block,
+// into part[b] (gridDim.x <= the length of part)
+static __global__ __launch_bounds__(256) void k_norm_partial(const cx<double> *__restrict__ src, long n,
+                                                      double *__restrict__ part) {
+    __shared__ double sb[4];
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const cx<double> z = src[i];
+        s += z.x * z.x + z.y * z.y;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sb[0] + sb[1] + sb[2] + sb[3];
+}
+
 // storage precision -> FP64 field with the inverse power-of-two scale
 static __global__ __launch_bounds__(256) void k_load_scaled(const cx<float> *__restrict__ src, cx<double> *__restrict__ dst,
                                                      long n3, long n2, long sps, long dps, int scale_exp) {

@@ -360,6 +360,10 @@ bool get_dispatch(int n, int precision, Dispatch *d) {
 
 // adds the elapsed time of every launch recorded since the last call to
 // its pass and counts one step
+// Any change of psi from outside the checkpointed run leaves the checkpoints
+// without the state they end in: they no longer serve a reverse sweep.
+inline void drop_checkpoints(ifdm_ctx *c) { c->ck_nsteps = -1; }
+
 int accumulate_profile(ifdm_ctx *c) {
     if (!c->prof) return 0;
     const size_t k = c->ev_pass.size();
@@ -553,6 +557,7 @@ int ifdm_set_psi(ifdm_ctx *c, const double *psi) {
     if (!c || !psi) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     c->pending = false;
+    drop_checkpoints(c);
     if (!c->mixed) {
         HIPCHK(c, plane_copy(c, c->psi, const_cast<double *>(psi), 2 * sizeof(double), c->n, true,
                              hipMemcpyHostToDevice));
@@ -638,6 +643,7 @@ int ifdm_kick(ifdm_ctx *c, double dPE, double dQI) {
     Dispatch d;
     get_dispatch(c->n, c->cfg.precision, &d);
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    drop_checkpoints(c);
     if (int rc0 = ifdm_internal_flush(c)) return rc0;
     int rc = d.kick(c, 0.5 * dPE * c->m_over_hbar, 0.5 * dQI * c->g_over_hbar);
     if (rc) return rc;
@@ -650,6 +656,7 @@ int ifdm_drift(ifdm_ctx *c, double dKE) {
     Dispatch d;
     get_dispatch(c->n, c->cfg.precision, &d);
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    drop_checkpoints(c);
     if (int rc0 = ifdm_internal_flush(c)) return rc0;
     int rc = d.drift(c, dKE);
     if (rc) return rc;
@@ -662,6 +669,7 @@ int ifdm_fft3d(ifdm_ctx *c, int sign) {
     Dispatch d;
     get_dispatch(c->n, c->cfg.precision, &d);
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    drop_checkpoints(c);
     if (int rc0 = ifdm_internal_flush(c)) return rc0;
     int rc = d.fft3d(c, sign);
     if (rc) return rc;
@@ -674,6 +682,7 @@ int ifdm_kdk_step(ifdm_ctx *c, const ifdm_dt *dt, ifdm_stats *st) {
     Dispatch d;
     get_dispatch(c->n, c->cfg.precision, &d);
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    drop_checkpoints(c);
     if (int rc0 = ifdm_internal_flush(c)) return rc0;
     const double aphi = 0.5 * dt->dPE * c->m_over_hbar, arho = 0.5 * dt->dQI * c->g_over_hbar;
     int rc = d.kdp(c, aphi, arho, dt->dKE);
@@ -688,6 +697,7 @@ int ifdm_step(ifdm_ctx *c, const ifdm_dt *dt, ifdm_stats *st) {
     Dispatch d;
     get_dispatch(c->n, c->cfg.precision, &d);
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    drop_checkpoints(c);
     const double aphi = 0.5 * dt->dPE * c->m_over_hbar, arho = 0.5 * dt->dQI * c->g_over_hbar;
     double kphi = aphi, krho = arho;
     if (c->pending) {
@@ -834,11 +844,20 @@ int ifdm_run_checkpointed(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, int K) 
         c->ck = nullptr;
     }
     c->ck_nsteps = -1;
+    c->ck_sched.clear();
     const int nck = nsteps / K + 1;
     if (hipMalloc(&c->ck, bytes * nck) != hipSuccess) {
         c->ck = nullptr;
         c->err = "hipMalloc failed for the adjoint checkpoints";
         return IFDM_E_NOMEM;
+    }
+    // phi of the entry state: the run is a function of psi_0 alone, whatever
+    // phi the context held before (the first half-kick reads it)
+    {
+        Dispatch d;
+        get_dispatch(c->n, c->cfg.precision, &d);
+        if ((rc = d.poisson(c))) return rc;
+        c->ev_pass.clear();
     }
     HIPCHK(c, hipMemcpyAsync(c->ck, c->psi, bytes, hipMemcpyDeviceToDevice, c->stream));
     for (int s = 0; s < nsteps; ++s) {
@@ -854,6 +873,7 @@ int ifdm_run_checkpointed(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, int K) 
     c->ck_k = K;
     c->ck_count = nck;
     c->ck_nsteps = nsteps;
+    c->ck_sched.assign(sched, sched + nsteps);
     return IFDM_OK;
 }
 
@@ -863,6 +883,7 @@ static int adjoint_sweep(ifdm_ctx *c, const ifdm_dt *sched, int nsteps) {
     get_dispatch(c->n, c->cfg.precision, &d);
     const size_t bytes = (c->mixed ? 2 * sizeof(float) : 2 * sizeof(double)) * (size_t)c->np;
     int rc;
+    c->ck_nsteps = -1;                                       // one sweep per forward run
     if ((rc = d.poisson(c))) return rc;                      // phi of the final state
     for (int n = nsteps - 1; n >= 0; --n) {
         const double aphi = 0.5 * sched[n].dPE * c->m_over_hbar, arho = 0.5 * sched[n].dQI * c->g_over_hbar;
@@ -884,10 +905,21 @@ static int adjoint_sweep(ifdm_ctx *c, const ifdm_dt *sched, int nsteps) {
     return IFDM_OK;
 }
 
-static int adjoint_prepare(ifdm_ctx *c, int nsteps) {
+// The checkpoints serve one reverse sweep: the sweep leaves psi_0 in psi
+// (not psi_T), so a second sweep would start from the wrong state.
+static int adjoint_prepare(ifdm_ctx *c, const ifdm_dt *sched, int nsteps) {
     if (!c->ck || c->ck_nsteps != nsteps) {
-        c->err = "ifdm_adjoint needs the checkpoints of an ifdm_run_checkpointed over the same steps";
+        c->err = "ifdm_adjoint needs the checkpoints of an ifdm_run_checkpointed over the same steps "
+                 "(each run serves one reverse sweep)";
         return IFDM_E_STATE;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        const ifdm_dt &a = c->ck_sched[s], &b = sched[s];
+        if (a.dKE != b.dKE || a.dPE != b.dPE || a.dQI != b.dQI) {
+            c->err = "ifdm_adjoint: schedule differs from the one of the last ifdm_run_checkpointed at step " +
+                     std::to_string(s);
+            return IFDM_E_STATE;
+        }
     }
     const size_t es = c->mixed ? sizeof(float) : sizeof(double);
     const size_t cs = 2 * es * (size_t)c->np, rs = es * (size_t)c->np;
@@ -909,7 +941,7 @@ int ifdm_adjoint(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, double *g) {
     if (!c || !sched || !g || nsteps < 0) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     int rc;
-    if ((rc = adjoint_prepare(c, nsteps))) return rc;
+    if ((rc = adjoint_prepare(c, sched, nsteps))) return rc;
     const size_t n2 = 2 * (size_t)c->n3;
     if (!c->mixed) {
         HIPCHK(c, plane_copy(c, c->adj_g, g, 2 * sizeof(double), c->n, true, hipMemcpyHostToDevice));
@@ -940,7 +972,7 @@ int ifdm_adjoint_device(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, void *g_d
     if (c->mixed) { c->err = "ifdm_adjoint_device runs in FP64 contexts (use ifdm_adjoint)"; return IFDM_E_STATE; }
     HIPCHK(c, hipSetDevice(c->cfg.device));
     int rc;
-    if ((rc = adjoint_prepare(c, nsteps))) return rc;
+    if ((rc = adjoint_prepare(c, sched, nsteps))) return rc;
     HIPCHK(c, plane_copy(c, c->adj_g, g_dev, 2 * sizeof(double), c->n, true, hipMemcpyDeviceToDevice));
     if ((rc = adjoint_sweep(c, sched, nsteps))) return rc;
     HIPCHK(c, plane_copy(c, c->adj_g, g_dev, 2 * sizeof(double), c->n, false, hipMemcpyDeviceToDevice));
@@ -952,6 +984,7 @@ int ifdm_set_psi_device(ifdm_ctx *c, const void *psi_dev) {
     if (!c || !psi_dev) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     c->pending = false;
+    drop_checkpoints(c);
     const long n2 = (long)c->n * c->n;
     if (!c->mixed) {
         HIPCHK(c, plane_copy(c, c->psi, const_cast<void *>(psi_dev), 2 * sizeof(double), c->n, true,
@@ -961,6 +994,18 @@ int ifdm_set_psi_device(ifdm_ctx *c, const void *psi_dev) {
         hipLaunchKernelGGL((k_store_scaled<float>), dim3(nb), dim3(256), 0, c->stream, (const cx<double> *)psi_dev,
                            (cx<float> *)c->psi, c->n3, n2, n2, c->ps, c->psi_scale);
         LAUNCHCHK(c);
+        // the FP64 mean density m_s <|psi|^2> that the complex-float Poisson
+        // source subtracts, from the FP64 input (as ifdm_set_psi does)
+        hipLaunchKernelGGL(k_norm_partial, dim3(IFDM_RED_BLOCKS), dim3(256), 0, c->stream,
+                           (const cx<double> *)psi_dev, c->n3, c->red_stage);
+        LAUNCHCHK(c);
+        std::vector<double> part(IFDM_RED_BLOCKS);
+        HIPCHK(c, hipMemcpyAsync(part.data(), c->red_stage, sizeof(double) * IFDM_RED_BLOCKS, hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        double norm = 0.0;
+        for (double x : part) norm += x;
+        c->rho_mean = c->ms * norm / (double)c->n3;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return IFDM_OK;

@@ -144,6 +144,7 @@ int ifdm_ic_finish(ifdm_ctx *c, double a, double *theta_out, float *vx_out, long
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const long n3 = c->n3;
     c->pending = false;
+    c->ck_nsteps = -1;   // psi replaced: the adjoint checkpoints no longer apply
     hipLaunchKernelGGL(k_ic_velnorm, dim3(grid_for(n3)), dim3(256), 0, c->stream, c->ic_rho, c->ic_vx, c->ic_vy,
                        c->ic_vz, n3, c->ic_pmass);
     HIPCHK(c, hipGetLastError());
@@ -357,6 +358,7 @@ int ifdm_read_snapshot(ifdm_ctx *c, const char *path, double *a_out, double *mfd
         }
     }
     c->pending = false;
+    c->ck_nsteps = -1;   // psi replaced: the adjoint checkpoints no longer apply
     if (c->mixed) c->rho_mean = c->ms * norm / (double)c->n3;
     if (a_out) *a_out = hdr[2];
     if (mfdm_out) *mfdm_out = hdr[0];

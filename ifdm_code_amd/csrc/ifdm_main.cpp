@@ -58,9 +58,18 @@ int main(int argc, char **argv) {
             if (i + 1 >= argc) { usage(); std::exit(2); }
             return argv[++i];
         };
-        if (o == "--precision") precision = std::strcmp(next(), "mixed") == 0 ? IFDM_MIXED : IFDM_FP64;
-        else if (o == "--no-a-triggers") a_triggers = 0;
-        else if (o == "--variant") variant = std::strcmp(next(), "mpi") == 0 ? 1 : 0;
+        if (o == "--precision") {
+            const std::string v = next();
+            if (v == "fp64") precision = IFDM_FP64;
+            else if (v == "mixed") precision = IFDM_MIXED;
+            else { std::fprintf(stderr, "ERROR: unknown precision %s\n", v.c_str()); usage(); return 2; }
+        } else if (o == "--no-a-triggers") a_triggers = 0;
+        else if (o == "--variant") {
+            const std::string v = next();
+            if (v == "stan") variant = 0;
+            else if (v == "mpi") variant = 1;
+            else { std::fprintf(stderr, "ERROR: unknown variant %s\n", v.c_str()); usage(); return 2; }
+        }
         else if (o == "--ref-maxphi-range") ref_range = 1;
         else if (o == "--ref-ghosts") ref_ghosts = 1;
         else if (o == "--nchunks") nchunks = std::atoi(next());
@@ -109,14 +118,14 @@ int main(int argc, char **argv) {
         ifdm_dt dt;
         double a_next;
         int which = 0;
-        if (check(ifdm_timestep_criteria(&cfg, a, &st, &dt, &a_next, &which), ctx, "ifdm_timestep_criteria")) break;
+        if ((rc = check(ifdm_timestep_criteria(&cfg, a, &st, &dt, &a_next, &which), ctx, "ifdm_timestep_criteria")))
+            break;
         std::printf("ai = %g a_next = %g da = %g n = %d ", a, a_next, a_next - a, which);
         a = a_next;                                                                   // :72
         int tw = -1, stop = 0;
         if (ifdm_output_trigger(variant, a_triggers, a, tn, twrite, t_stop, &trig, &tw, &stop) == 1) {
             const std::string out = opath + "output" + std::to_string(tw) + ".dat";
-            rc = ifdm_write_snapshot(ctx, out.c_str(), a, mfdm, g_in);
-            if (check(rc, ctx, "ifdm_write_snapshot")) break;
+            if ((rc = check(ifdm_write_snapshot(ctx, out.c_str(), a, mfdm, g_in), ctx, "ifdm_write_snapshot"))) break;
             std::printf("write %s at step %d ", out.c_str(), tn + 1);
             if (stop) { std::printf("\n"); break; }
         }

@@ -89,6 +89,40 @@ def test_adjoint_against_oracle_finite_differences():
         assert abs(ad - fd) <= 1e-6 * abs(fd)
 
 
+def test_checkpoints_serve_one_matching_sweep():
+    """ifdm_run_checkpointed solves Poisson for the entry state itself, and
+    its checkpoints serve exactly one reverse sweep over the same schedule:
+    a second sweep, a sweep over another schedule, or a sweep after psi was
+    replaced is refused (IFDM_E_STATE) instead of returning the gradient of
+    another function."""
+    n, steps = 16, 6
+    F, O, rp, cfg, psi0, sched, rho_t, rbar = _setup(n, steps, 0.0, seed=31)
+    with F.Solver(cfg) as s:
+        s.set_psi(psi0)
+        s.poisson()
+        s.run(sched)
+        fresh = s.get_psi()
+        # stale potential from another field on entry
+        s.set_psi(make_psi(n, 98, cfg.mfdm_solar))
+        s.poisson()
+        s.set_psi(psi0)
+        s.run_checkpointed(sched, 2)
+        psiT = s.get_psi()
+        assert rel_l2(psiT, fresh) < 1e-14
+        seed = _seed(psiT, rho_t, rbar)
+        other = [(a * 1.01, b, c) for a, b, c in sched]
+        with pytest.raises(F.IfdmError, match="schedule differs"):
+            s.adjoint(other, seed)
+        g = s.adjoint(sched, seed)
+        assert np.isfinite(g).all()
+        with pytest.raises(F.IfdmError, match="one reverse sweep"):
+            s.adjoint(sched, seed)
+        s.run_checkpointed(sched, 2)
+        s.set_psi(psi0)
+        with pytest.raises(F.IfdmError, match="one reverse sweep"):
+            s.adjoint(sched, seed)
+
+
 def test_adjoint_is_exact_for_a_pure_drift():
     """With dPE = dQI = 0 the step is the unitary drift: the adjoint of
     g_T is D^H g_T, i.e. the drift with -dKE applied to g_T."""
@@ -123,24 +157,41 @@ from tests.test_gpu_adjoint import _setup, _seed
 from tests.fields import rel_l2
 n, steps = 16, 6
 F_, O, rp, cfg, psi0, sched, rho_t, rbar = _setup(n, steps, 0.0, seed=21)
+from tests.fields import make_psi
 with F.Solver(cfg) as s:
-    s.set_psi(psi0)
+    # the solver holds another field and its potential: evolve() must not
+    # depend on them (it solves Poisson for psi0 itself)
+    s.set_psi(make_psi(n, 99, cfg.mfdm_solar))
     s.poisson()
     p0 = torch.tensor(psi0, dtype=torch.complex128, device="cuda:0", requires_grad=True)
     pT = evolve(p0, s, sched, 3)
     rt = torch.tensor(rho_t, device="cuda:0")
     loss = (((pT.abs() ** 2) - rt) / rbar).pow(2).sum()
-    loss.backward()
-    g_torch = p0.grad.detach().cpu().numpy()
+    loss.backward(retain_graph=True)
+    g_torch = p0.grad.detach().cpu().numpy().copy()
+    # a second backward through the same graph has no checkpoints left: it
+    # must raise rather than return another function's gradient
+    try:
+        loss.backward()
+        second = "returned"
+    except RuntimeError as e:
+        second = "raised"
     s.set_psi(psi0)
-    s.poisson()
     s.run_checkpointed(sched, 3)
     psiT = s.get_psi()
     g_lib = s.adjoint(sched, _seed(psiT, rho_t, rbar))
+# the same forward on a fresh solver
+with F.Solver(cfg) as s2:
+    s2.set_psi(psi0)
+    s2.poisson()
+    s2.run(sched)
+    fresh = s2.get_psi()
+e0 = rel_l2(pT.detach().cpu().numpy(), fresh)
 e1 = rel_l2(pT.detach().cpu().numpy(), psiT)
 e2 = rel_l2(g_torch, g_lib)
-print("forward", e1, "gradient", e2, flush=True)
-assert e1 < 1e-14 and e2 < 1e-12
+print("forward vs fresh", e0, "forward", e1, "gradient", e2, "second backward", second, flush=True)
+assert e0 < 1e-14 and e1 < 1e-14 and e2 < 1e-12
+assert second == "raised"
 """
 
 
