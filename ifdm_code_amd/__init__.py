@@ -281,7 +281,8 @@ class Solver:
         changes with every such run (the autograd binding checks it)."""
         arr, m = self._sched(schedule)
         self._chk(lib().ifdm_run_checkpointed(self._h, arr, m, int(K)), "ifdm_run_checkpointed")
-        self.ck_serial = getattr(self, "ck_serial", 0) + 1
+        self._ck_runs = getattr(self, "_ck_runs", 0) + 1
+        self.ck_serial = self._ck_runs
         return self.ck_serial
 
     def adjoint(self, schedule, g_T: np.ndarray) -> np.ndarray:
@@ -311,8 +312,12 @@ class Solver:
         return info.as_dict()
 
     def ic_from_particles(self, chunks, box: float, pmass: float, a: float, ref_ghosts: bool = False,
-                          want_theta: bool = False):
-        """chunks: iterable of (pos, vel) float32 arrays of shape (np, 3)."""
+                          want_theta: bool = False, want_vx: bool = False):
+        """chunks: iterable of (pos, vel) float32 arrays of shape (np, 3).
+        Returns (theta or None, skipped), and with want_vx also the
+        mass-weighted float vx grid (what voutput.dat holds).  The deposit
+        follows the "ic_deposit" option (set_option): 0, the default, the
+        reference's order and float velocity grids; 1, FP64 atomics."""
         self._chk(lib().ifdm_ic_begin(self._h, box, pmass), "ifdm_ic_begin")
         fp = ctypes.POINTER(ctypes.c_float)
         for pos, vel in chunks:
@@ -322,9 +327,13 @@ class Solver:
                                             int(ref_ghosts)), "ifdm_ic_deposit")
         n = self.cfg.n
         theta = np.empty((n, n, n), np.complex128) if want_theta else None
+        vx = np.empty((n, n, n), np.float32) if want_vx else None
         sk = ctypes.c_longlong()
-        self._chk(lib().ifdm_ic_finish(self._h, a, _dp(theta.view(np.float64)) if want_theta else None, None,
-                                       ctypes.byref(sk)), "ifdm_ic_finish")
+        self._chk(lib().ifdm_ic_finish(self._h, a, _dp(theta.view(np.float64)) if want_theta else None,
+                                       vx.ctypes.data_as(fp) if want_vx else None, ctypes.byref(sk)),
+                  "ifdm_ic_finish")
+        if want_vx:
+            return theta, sk.value, vx
         return theta, sk.value
 
     def set_rhoc(self, rhoc: float):

@@ -48,8 +48,14 @@ struct ifdm_ctx {
     // second half-kick of the last ifdm_step, not yet applied
     bool pending = false;
     double pend_aphi = 0.0, pend_arho = 0.0;
-    // initial-condition grids (ifdm_ic_begin .. ifdm_ic_finish), FP64
-    double *ic_rho = nullptr, *ic_vx = nullptr, *ic_vy = nullptr, *ic_vz = nullptr;
+    // initial-condition grids (ifdm_ic_begin .. ifdm_ic_finish): the FP64
+    // density and the velocity grids, float (ic_f32, the reference-order
+    // deposit) or FP64 (the atomic deposit)
+    double *ic_rho = nullptr;
+    void *ic_vx = nullptr, *ic_vy = nullptr, *ic_vz = nullptr;
+    bool ic_f32 = true;
+    int ic_mode = 0;                  // "ic_deposit" option: 0 reference order, 1 FP64 atomics
+    unsigned *ic_first = nullptr, *ic_last = nullptr;   // per-cell ranges of the ordered deposit
     unsigned long long *ic_skipped = nullptr;
     double ic_box = 0.0, ic_pmass = 0.0;
     long long ic_npart = 0;

@@ -404,7 +404,7 @@ void release(ifdm_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &e : c->ev_pool)
         if (e) (void)hipEventDestroy(e);
-    void *icb[] = {c->ic_rho, c->ic_vx, c->ic_vy, c->ic_vz, c->ic_skipped};
+    void *icb[] = {c->ic_rho, c->ic_vx, c->ic_vy, c->ic_vz, c->ic_skipped, c->ic_first, c->ic_last};
     for (void *b : icb)
         if (b) (void)hipFree(b);
     void *bufs[] = {c->psi, c->phi, c->shat, c->tw, c->k2, c->part_rho, c->part_phi, c->red, c->red_stage, c->dtab,
@@ -1071,6 +1071,12 @@ int ifdm_set_option(ifdm_ctx *c, const char *name, long value) {
     if (std::strcmp(name, "persistent") == 0) {
         if (value < 0 || value > 31) { c->err = "persistent must be a mask in 0..31"; return IFDM_E_ARG; }
         c->persistent = (int)value;
+        return IFDM_OK;
+    }
+    if (std::strcmp(name, "ic_deposit") == 0) {
+        if (value != 0 && value != 1) { c->err = "ic_deposit must be 0 (reference order) or 1 (FP64 atomics)"; return IFDM_E_ARG; }
+        if (c->ic_rho) { c->err = "ic_deposit cannot change between ifdm_ic_begin and ifdm_ic_finish"; return IFDM_E_STATE; }
+        c->ic_mode = (int)value;
         return IFDM_OK;
     }
     c->err = std::string("unknown option ") + name;

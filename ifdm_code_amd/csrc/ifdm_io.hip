@@ -17,6 +17,8 @@
 #include "ctx.h"
 #include "ic_kernels.h"
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 using namespace ifdm;
 
 namespace {
@@ -24,11 +26,70 @@ namespace {
 unsigned grid_for(long n) { return (unsigned)std::min<long>((n + 255) / 256, 1L << 16); }
 
 void free_ic(ifdm_ctx *c) {
-    void *b[] = {c->ic_rho, c->ic_vx, c->ic_vy, c->ic_vz, c->ic_skipped};
+    void *b[] = {c->ic_rho, c->ic_vx, c->ic_vy, c->ic_vz, c->ic_skipped, c->ic_first, c->ic_last};
     for (void *p : b)
         if (p) (void)hipFree(p);
-    c->ic_rho = c->ic_vx = c->ic_vy = c->ic_vz = nullptr;
+    c->ic_rho = nullptr;
+    c->ic_vx = c->ic_vy = c->ic_vz = nullptr;
+    c->ic_first = c->ic_last = nullptr;
     c->ic_skipped = nullptr;
+}
+
+// Device buffers that live for one call; freed on every return path.
+struct DevBufs {
+    std::vector<void *> p;
+    ~DevBufs() {
+        for (void *q : p)
+            if (q) (void)hipFree(q);
+    }
+    template <typename X> bool get(X **out, size_t bytes) {
+        void *q = nullptr;
+        if (hipMalloc(&q, bytes ? bytes : 1) != hipSuccess) return false;
+        p.push_back(q);
+        *out = (X *)q;
+        return true;
+    }
+};
+
+// The reference-order deposit of one ifdm_ic_deposit call: the sequence of
+// ng ghost copies then np particles (pos, vel on the device), in batches of
+// at most 2^27 entries, each sorted by base cell and gathered per target cell.
+int deposit_ordered(ifdm_ctx *c, const float *dpos, const float *dvel, const CicOrd &a) {
+    const long long total = a.ng + a.np;
+    if (total == 0) return IFDM_OK;
+    if (total >= (1ll << 32) - 1) { c->err = "too many particles in one ifdm_ic_deposit call"; return IFDM_E_ARG; }
+    const unsigned n3 = (unsigned)c->n3;
+    const long long batch = std::min<long long>(total, 1ll << 27);
+    unsigned end_bit = 1;
+    while ((1ull << end_bit) <= (unsigned long long)n3) ++end_bit;   // keys <= n3
+    DevBufs B;
+    unsigned *kin, *kout, *vin, *vout;
+    if (!B.get(&kin, 4 * batch) || !B.get(&kout, 4 * batch) || !B.get(&vin, 4 * batch) || !B.get(&vout, 4 * batch)) {
+        c->err = "hipMalloc failed for the deposit's sort buffers";
+        return IFDM_E_NOMEM;
+    }
+    size_t tmp_bytes = 0;
+    HIPCHK(c, rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, vin, vout, (size_t)batch, 0, end_bit, c->stream));
+    void *tmp;
+    if (!B.get(&tmp, tmp_bytes)) { c->err = "hipMalloc failed for the deposit's sort storage"; return IFDM_E_NOMEM; }
+    for (long long s0 = 0; s0 < total; s0 += batch) {
+        const long long m = std::min(batch, total - s0);
+        const unsigned nb = (unsigned)((m + 255) / 256);
+        hipLaunchKernelGGL(k_cico_keys, dim3(nb), dim3(256), 0, c->stream, dpos, a, s0, m, kin, vin, c->ic_skipped);
+        LAUNCHCHK(c);
+        size_t tb = tmp_bytes;
+        HIPCHK(c, rocprim::radix_sort_pairs(tmp, tb, kin, kout, vin, vout, (size_t)m, 0, end_bit, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->ic_first, 0, sizeof(unsigned) * n3, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->ic_last, 0, sizeof(unsigned) * n3, c->stream));
+        hipLaunchKernelGGL(k_cico_bounds, dim3(nb), dim3(256), 0, c->stream, kout, m, n3, c->ic_first, c->ic_last);
+        LAUNCHCHK(c);
+        hipLaunchKernelGGL((k_cico_gather<float>), dim3(grid_for(c->n3)), dim3(256), 0, c->stream, dpos, dvel, a, vout,
+                           c->ic_first, c->ic_last, c->ic_rho, (float *)c->ic_vx, (float *)c->ic_vy,
+                           (float *)c->ic_vz);
+        LAUNCHCHK(c);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return IFDM_OK;
 }
 
 struct File {
@@ -63,15 +124,20 @@ int ifdm_ic_begin(ifdm_ctx *c, double box, double pmass) {
     if (!c || !(box > 0) || !(pmass > 0)) return IFDM_E_ARG;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     free_ic(c);
-    const size_t bytes = sizeof(double) * (size_t)c->n3;
-    double **g[] = {&c->ic_rho, &c->ic_vx, &c->ic_vy, &c->ic_vz};
-    for (double **p : g) {
-        if (hipMalloc((void **)p, bytes) != hipSuccess) {
+    c->ic_f32 = c->ic_mode == 0;
+    const size_t n3 = (size_t)c->n3, vb = (c->ic_f32 ? sizeof(float) : sizeof(double)) * n3;
+    struct { void **p; size_t bytes; } g[] = {{(void **)&c->ic_rho, sizeof(double) * n3},
+                                              {&c->ic_vx, vb}, {&c->ic_vy, vb}, {&c->ic_vz, vb},
+                                              {(void **)&c->ic_first, c->ic_f32 ? sizeof(unsigned) * n3 : 0},
+                                              {(void **)&c->ic_last, c->ic_f32 ? sizeof(unsigned) * n3 : 0}};
+    for (auto &b : g) {
+        if (!b.bytes) continue;
+        if (hipMalloc(b.p, b.bytes) != hipSuccess) {
             free_ic(c);
             c->err = "hipMalloc failed for the initial-condition grids";
             return IFDM_E_NOMEM;
         }
-        HIPCHK(c, hipMemsetAsync(*p, 0, bytes, c->stream));
+        HIPCHK(c, hipMemsetAsync(*b.p, 0, b.bytes, c->stream));
     }
     HIPCHK(c, hipMalloc((void **)&c->ic_skipped, sizeof(unsigned long long)));
     HIPCHK(c, hipMemsetAsync(c->ic_skipped, 0, sizeof(unsigned long long), c->stream));
@@ -103,6 +169,36 @@ int ifdm_ic_deposit(ifdm_ctx *c, const float *pos, const float *vel, long long n
             else break;
         }
     }
+    if (c->ic_f32) {
+        // reference order: the whole call's particles on the device (the
+        // ghosts index into the chunk's end)
+        DevBufs B;
+        float *dp, *dv;
+        if (!B.get(&dp, sizeof(float) * 3 * (size_t)np) || !B.get(&dv, sizeof(float) * 3 * (size_t)np)) {
+            c->err = "hipMalloc failed for the particles";
+            return IFDM_E_NOMEM;
+        }
+        HIPCHK(c, hipMemcpy(dp, pos, sizeof(float) * 3 * (size_t)np, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(dv, vel, sizeof(float) * 3 * (size_t)np, hipMemcpyHostToDevice));
+        std::vector<double> xg(c->n);
+        for (int i = 0; i < c->n; ++i) xg[i] = i * a.lg + 0.5 * a.lg;                  // src/gadget_IC.h:333
+        double *dxg;
+        if (!B.get(&dxg, sizeof(double) * c->n)) { c->err = "hipMalloc failed"; return IFDM_E_NOMEM; }
+        HIPCHK(c, hipMemcpy(dxg, xg.data(), sizeof(double) * c->n, hipMemcpyHostToDevice));
+        CicOrd o;
+        o.n = c->n;
+        o.box = a.box;
+        o.lg = a.lg;
+        o.hl = 0.5 * a.lg;
+        o.pmass = a.pmass;
+        o.l3 = a.l3;
+        o.xg = dxg;
+        o.np = np;
+        o.ng = np - ghost0;
+        const int rc = deposit_ordered(c, dp, dv, o);
+        if (!rc) c->ic_npart += np + (np - ghost0);
+        return rc;
+    }
     const long long batch = 1ll << 24;
     float *dp = nullptr, *dv = nullptr;
     const long long cap = std::min<long long>(std::max<long long>(np, 1), batch);
@@ -123,7 +219,8 @@ int ifdm_ic_deposit(ifdm_ctx *c, const float *pos, const float *vel, long long n
                 break;
             }
             hipLaunchKernelGGL(k_cic, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c->stream, dp, dv, (long)m, a,
-                               c->ic_rho, c->ic_vx, c->ic_vy, c->ic_vz, c->ic_skipped);
+                               c->ic_rho, (double *)c->ic_vx, (double *)c->ic_vy, (double *)c->ic_vz,
+                               c->ic_skipped);
             if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
                 rc = IFDM_E_HIP;
                 c->err = "CIC deposit kernel failed";
@@ -145,14 +242,25 @@ int ifdm_ic_finish(ifdm_ctx *c, double a, double *theta_out, float *vx_out, long
     const long n3 = c->n3;
     c->pending = false;
     c->ck_nsteps = -1;   // psi replaced: the adjoint checkpoints no longer apply
-    hipLaunchKernelGGL(k_ic_velnorm, dim3(grid_for(n3)), dim3(256), 0, c->stream, c->ic_rho, c->ic_vx, c->ic_vy,
-                       c->ic_vz, n3, c->ic_pmass);
+    if (c->ic_f32)
+        hipLaunchKernelGGL((k_ic_velnorm<float>), dim3(grid_for(n3)), dim3(256), 0, c->stream, c->ic_rho,
+                           (float *)c->ic_vx, (float *)c->ic_vy, (float *)c->ic_vz, n3, c->ic_pmass);
+    else
+        hipLaunchKernelGGL((k_ic_velnorm<double>), dim3(grid_for(n3)), dim3(256), 0, c->stream, c->ic_rho,
+                           (double *)c->ic_vx, (double *)c->ic_vy, (double *)c->ic_vz, n3, c->ic_pmass);
     HIPCHK(c, hipGetLastError());
     if (vx_out) {
-        std::vector<double> tmp((size_t)n3);
-        HIPCHK(c, hipMemcpyAsync(tmp.data(), c->ic_vx, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (long i = 0; i < n3; ++i) vx_out[i] = (float)tmp[i];
+        // write_binary_vg writes the float grid after the normalisation
+        // (src/gadget_IC.h:656, src/write3D.h:24-53)
+        if (c->ic_f32) {
+            HIPCHK(c, hipMemcpyAsync(vx_out, c->ic_vx, sizeof(float) * n3, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        } else {
+            std::vector<double> tmp((size_t)n3);
+            HIPCHK(c, hipMemcpyAsync(tmp.data(), c->ic_vx, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            for (long i = 0; i < n3; ++i) vx_out[i] = (float)tmp[i];
+        }
     }
     // theta work buffer (psi's plane stride): psi itself in FP64 mode
     void *buf = c->psi;
@@ -162,8 +270,14 @@ int ifdm_ic_finish(ifdm_ctx *c, double a, double *theta_out, float *vx_out, long
     }
     const double dx = 2.0 * c->cfg.xmax / c->n;                          // par.dx, src/struct3D.h:52
     const double unit = 1e3 / m2Mpc * s2Myr;                              // src/gadget_IC.h:171
-    hipLaunchKernelGGL(k_ic_divergence, dim3(grid_for(n3)), dim3(256), 0, c->stream, c->ic_vx, c->ic_vy, c->ic_vz,
-                       (cx<double> *)buf, c->n, c->ps, unit / dx);
+    if (c->ic_f32)
+        hipLaunchKernelGGL((k_ic_divergence<float>), dim3(grid_for(n3)), dim3(256), 0, c->stream,
+                           (const float *)c->ic_vx, (const float *)c->ic_vy, (const float *)c->ic_vz,
+                           (cx<double> *)buf, c->n, c->ps, unit, dx);
+    else
+        hipLaunchKernelGGL((k_ic_divergence<double>), dim3(grid_for(n3)), dim3(256), 0, c->stream,
+                           (const double *)c->ic_vx, (const double *)c->ic_vy, (const double *)c->ic_vz,
+                           (cx<double> *)buf, c->n, c->ps, unit, dx);
     int rc = hipGetLastError() == hipSuccess ? IFDM_OK : IFDM_E_HIP;
     if (!rc) rc = ifdm_internal_theta_solve(c, buf, -std::pow(a, 3. / 2));   // src/gadget_IC.h:233
     if (!rc && theta_out) {

@@ -13,6 +13,10 @@
 //   --ref-maxphi-range       max|phi| over the reference's one-rank scan range
 //   --ref-ghosts             deposit the x-sorted chunks' trailing particles twice, as at one rank
 //   --nchunks K              IC files per snapshot (default 4)
+//   --ic-deposit reference|atomic
+//                            CIC in the reference's particle order with float
+//                            velocity grids (default; the reference's bits) or
+//                            unordered FP64 atomics
 //   --device D               HIP device
 // Uses only the C ABI of libifdm.so.
 #include <cmath>
@@ -28,7 +32,7 @@ static void usage() {
     std::fprintf(stderr,
                  "usage: ifdm mfdm g xmax res t_write t_stop ti opath icpath rerun "
                  "[--precision fp64|mixed] [--no-a-triggers] [--variant stan|mpi] [--ref-maxphi-range] "
-                 "[--ref-ghosts] [--nchunks K] [--device D]\n");
+                 "[--ref-ghosts] [--nchunks K] [--ic-deposit reference|atomic] [--device D]\n");
 }
 
 static int check(int rc, ifdm_ctx *ctx, const char *what) {
@@ -52,6 +56,7 @@ int main(int argc, char **argv) {
     const std::string opath = argv[8], icpath = argv[9];
     const bool rerun = std::atof(argv[10]) == 1;
     int precision = IFDM_FP64, a_triggers = 1, variant = 0, ref_range = 0, ref_ghosts = 0, nchunks = 4, device = 0;
+    int ic_atomic = 0;
     for (int i = 11; i < argc; ++i) {
         const std::string o = argv[i];
         auto next = [&](void) -> const char * {
@@ -73,6 +78,12 @@ int main(int argc, char **argv) {
         else if (o == "--ref-maxphi-range") ref_range = 1;
         else if (o == "--ref-ghosts") ref_ghosts = 1;
         else if (o == "--nchunks") nchunks = std::atoi(next());
+        else if (o == "--ic-deposit") {
+            const std::string v = next();
+            if (v == "reference") ic_atomic = 0;
+            else if (v == "atomic") ic_atomic = 1;
+            else { std::fprintf(stderr, "ERROR: unknown deposit %s\n", v.c_str()); usage(); return 2; }
+        }
         else if (o == "--device") device = std::atoi(next());
         else { usage(); return 2; }
     }
@@ -92,6 +103,7 @@ int main(int argc, char **argv) {
                 mfdm, g_in, cfg.xmax, 2 * cfg.xmax, res, twrite, t_stop, ti, opath.c_str());
     ifdm_ctx *ctx = nullptr;
     if (check(ifdm_create(&cfg, &ctx), nullptr, "ifdm_create")) return 1;
+    if (check(ifdm_set_option(ctx, "ic_deposit", ic_atomic), ctx, "ifdm_set_option")) { ifdm_destroy(ctx); return 1; }
     const time_t t0 = time(nullptr);
     double a = 0.0;
     int rc = 0;

@@ -142,7 +142,15 @@ int ifdm_simulate(ifdm_ctx *ctx, double a0, int nsteps, double *a_out, ifdm_dt *
 #define IFDM_NPASS 9
 int ifdm_time_passes(ifdm_ctx *ctx, const ifdm_dt *dt, int iters, float *ms_out);
 
-/* Launch-schedule options (no effect on results beyond round-off):
+/* Options.
+ *   "ic_deposit": 0 (default) deposits the initial conditions the way the
+ *   reference's CIC_scheme does at one rank, bit for bit: each cell's
+ *   contributions in the reference's particle order, the density in FP64
+ *   and the three velocity grids in float (src/gadget_IC.h:112-150,583),
+ *   the divergence in float (:170), so two runs give the same psi; 1 uses
+ *   unordered FP64 atomic adds into FP64 grids (round-off differences from
+ *   run to run).  Set before ifdm_ic_begin / ifdm_read_gadget.
+ * Launch-schedule options (no effect on results beyond round-off):
  *   "persistent" mask: bit 0 the y passes on psi (drift only; the kick pass
  *   always runs one tile per work-group), bit 1 the x passes on psi, bit 2
  *   the forward y pass on the half spectrum (F), bit 3 its x pass (G), bit 4
@@ -187,11 +195,16 @@ int ifdm_power_spectrum(ifdm_ctx *ctx, int nbins, double *k_mean, double *pk, do
 
 /* Forward run of nsteps steps with the given schedule (as ifdm_run) that
  * keeps psi every K steps (step 0, K, 2K, ...) on the device for
- * ifdm_adjoint.  In mixed contexts the checkpoints are FP32 and
- * ifdm_adjoint holds g as 2^-s g in FP32 (the sweep is linear in g). */
+ * ifdm_adjoint.  phi is solved for the current psi first, so the run is a
+ * function of psi alone.  In mixed contexts the checkpoints are FP32 and
+ * ifdm_adjoint holds g as 2^-s g in FP32 (the sweep is linear in g).  The
+ * checkpoints serve one reverse sweep; any call that changes psi (set,
+ * step, kick, drift, transform, IC, snapshot read) discards them. */
 int ifdm_run_checkpointed(ifdm_ctx *ctx, const ifdm_dt *sched, int nsteps, int K);
 
-/* Reverse sweep of the last ifdm_run_checkpointed (same sched, nsteps).
+/* Reverse sweep of the last ifdm_run_checkpointed (the same sched, checked
+ * entry by entry; IFDM_E_STATE otherwise or when the checkpoints were used
+ * or discarded).
  * On entry g holds dL/dRe psi_T + i dL/dIm psi_T for a real loss L of the
  * final state (host N^3 complex128, reference layout); on return it holds
  * dL/dRe psi_0 + i dL/dIm psi_0.  The schedule is frozen (dt does not

@@ -266,11 +266,13 @@ def simulate(rp: RunParams, psi0: np.ndarray, a0: float, nsteps: int, ref_maxphi
 
 
 def ic_from_particles(rp: RunParams, chunks, box: float, pmass: float, a: float, ref_p1: bool = False,
-                      f32: bool = False):
+                      f32: bool = False, grids: bool = False):
     """read_gadget after the file reads (src/gadget_IC.h:610-672): CIC of
     every chunk, mass-weighted velocities, velocity-divergence phase and
     psi.  box kpc/h, pmass Msun/h.  f32 reproduces the reference's float
-    velocity grids.  Returns (psi, theta, skipped)."""
+    velocity grids.  Returns (psi, theta, skipped), and with grids=True
+    also (rho, vx, vy, vz): the deposited density and the mass-weighted
+    velocity grids (vx is what write_binary_vg writes, src/write3D.h:24-53)."""
     n = rp.n
     rho = np.zeros((n, n, n))
     vs = [np.zeros((n, n, n)) for _ in range(3)]
@@ -287,6 +289,8 @@ def ic_from_particles(rp: RunParams, chunks, box: float, pmass: float, a: float,
     lib().oracle_ic_assemble(ctypes.byref(cp), pmass, a, rp.dx, h_0, m2Mpc, s2Myr, int(f32), _dp(rho),
                              _dp(vs[0]), _dp(vs[1]), _dp(vs[2]), _dp(psi.view(np.float64)),
                              _dp(theta.view(np.float64)))
+    if grids:
+        return psi, theta, skipped, (rho, vs[0], vs[1], vs[2])
     return psi, theta, skipped
 
 

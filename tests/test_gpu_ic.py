@@ -24,24 +24,91 @@ def _mods():
     return F, icgen, O
 
 
-@pytest.mark.parametrize("n,ghosts", [(16, False), (32, False), (32, True)])
-def test_ic_matches_oracle(n, ghosts):
-    F, icgen, O = _mods()
-    _say(f"[ic] n={n} ghosts={ghosts}")
-    pos, vel, pm, a = icgen.zeldovich(n, 10.0, seed=n)
+def _sorted_chunks(icgen, n, seed, box_mpc_h=10.0):
+    pos, vel, pm, a = icgen.zeldovich(n, box_mpc_h, seed=seed)
     order = np.argsort(pos[:, 0], kind="stable")
     pos, vel = pos[order], vel[order]
     per = n ** 3 // 4
-    chunks = [(pos[c * per:(c + 1) * per], vel[c * per:(c + 1) * per]) for c in range(4)]
+    return [(pos[c * per:(c + 1) * per], vel[c * per:(c + 1) * per]) for c in range(4)], pm, a
+
+
+@pytest.mark.parametrize("n,ghosts", [(16, False), (32, False), (32, True), (64, True)])
+def test_ic_matches_reference_float_deposit(n, ghosts):
+    """The default deposit reproduces the reference's arithmetic: the
+    density accumulated in FP64 and the velocity grids in float, each cell's
+    contributions in the reference's particle order, the divergence in float
+    (src/gadget_IC.h:112-150,170,583,645-651).  The mass-weighted float vx
+    grid equals the oracle's bit for bit; theta and psi_0 differ only by the
+    FFT round-off of the phase solve."""
+    F, icgen, O = _mods()
+    _say(f"[ic reference order] n={n} ghosts={ghosts}")
+    chunks, pm, a = _sorted_chunks(icgen, n, n)
+    rp = O.RunParams(n=n, xmax_in=5.0)
+    psi_o, th_o, _, (rho_o, vx_o, vy_o, vz_o) = O.ic_from_particles(rp, chunks, 10.0e3, pm * 1e10, a,
+                                                                    ref_p1=ghosts, f32=True, grids=True)
+    cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n)
+    with F.Solver(cfg) as s:
+        th, skipped, vx = s.ic_from_particles(chunks, 10.0e3, pm * 1e10, a, ref_ghosts=ghosts, want_theta=True,
+                                             want_vx=True)
+        psi = s.get_psi()
+    assert skipped == 0
+    assert np.array_equal(vx, vx_o.astype(np.float32))
+    e_th, e_psi = rel_l2(th, th_o), rel_l2(psi, psi_o)
+    _say(f"   theta {e_th:.3e} psi {e_psi:.3e}")
+    assert e_th < 1e-12
+    assert e_psi < 1e-12
+
+
+@pytest.mark.parametrize("n,ghosts", [(16, False), (32, True)])
+def test_ic_fp64_atomic_deposit_matches_oracle(n, ghosts):
+    """The FP64-atomic deposit (ic_deposit = 1) against the oracle's FP64
+    grids: unordered sums, round-off agreement."""
+    F, icgen, O = _mods()
+    _say(f"[ic atomic] n={n} ghosts={ghosts}")
+    chunks, pm, a = _sorted_chunks(icgen, n, n)
     rp = O.RunParams(n=n, xmax_in=5.0)
     psi_o, th_o, _ = O.ic_from_particles(rp, chunks, 10.0e3, pm * 1e10, a, ref_p1=ghosts)
     cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n)
     with F.Solver(cfg) as s:
+        s.set_option("ic_deposit", 1)
         th, skipped = s.ic_from_particles(chunks, 10.0e3, pm * 1e10, a, ref_ghosts=ghosts, want_theta=True)
         psi = s.get_psi()
     assert skipped == 0
     assert rel_l2(th, th_o) < 1e-11
     assert rel_l2(psi, psi_o) < 1e-12
+
+
+def test_ic_reference_deposit_is_deterministic():
+    """Two runs of the default deposit give psi_0 bit for bit (the order of
+    every cell's additions is fixed), also with ghosts and several chunks;
+    particles outside the box are counted and left out, as by the oracle."""
+    F, icgen, O = _mods()
+    n = 32
+    chunks, pm, a = _sorted_chunks(icgen, n, 77)
+    # three particles the reference cannot place (x below the wrapped half
+    # cell, x and y beyond the last cell): skipped; a y below the box wraps
+    # (src/gadget_IC.h:67-80) and is deposited
+    p0, v0 = chunks[0]
+    p0 = p0.copy()
+    p0[5, 0] = -500.0
+    p0[9, 0] = 10.2e3
+    p0[12, 1] = 10.2e3
+    p0[20, 1] = -400.0
+    chunks[0] = (p0, v0)
+    cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n)
+    outs = []
+    for _ in range(2):
+        with F.Solver(cfg) as s:
+            _th, skipped, vx = s.ic_from_particles(chunks, 10.0e3, pm * 1e10, a, ref_ghosts=True, want_vx=True)
+            outs.append((s.get_psi(), vx, skipped))
+    assert outs[0][2] == 3 and outs[1][2] == 3
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    rp = O.RunParams(n=n, xmax_in=5.0)
+    psi_o, _th, sk_o, grids = O.ic_from_particles(rp, chunks, 10.0e3, pm * 1e10, a, ref_p1=True, f32=True,
+                                                  grids=True)
+    assert sk_o == 3
+    assert np.array_equal(outs[0][1], grids[1].astype(np.float32))
+    assert rel_l2(outs[0][0], psi_o) < 1e-12
 
 
 def test_ic_mixed_precision():
@@ -50,7 +117,7 @@ def test_ic_mixed_precision():
     _say(f"[ic mixed] n={n}")
     pos, vel, pm, a = icgen.zeldovich(n, 10.0, seed=2)
     rp = O.RunParams(n=n)
-    psi_o, _, _ = O.ic_from_particles(rp, [(pos, vel)], 10.0e3, pm * 1e10, a)
+    psi_o, _, _ = O.ic_from_particles(rp, [(pos, vel)], 10.0e3, pm * 1e10, a, f32=True)
     cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n, precision=F.MIXED)
     with F.Solver(cfg) as s:
         s.ic_from_particles([(pos, vel)], 10.0e3, pm * 1e10, a)
@@ -65,7 +132,7 @@ def test_read_gadget(tmp_path):
     pos, vel, pm, a = icgen.zeldovich(n, 10.0, seed=4)
     chunks = icgen.write_gadget(str(tmp_path / "ic" / "sorted"), pos, vel, pm, a, 10.0e3)
     rp = O.RunParams(n=n)
-    psi_o, _, _ = O.ic_from_particles(rp, chunks, 10.0e3, pm * 1e10, a)
+    psi_o, _, _, grids = O.ic_from_particles(rp, chunks, 10.0e3, pm * 1e10, a, f32=True, grids=True)
     cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n)
     vout = str(tmp_path / "voutput.dat")
     with F.Solver(cfg) as s:
@@ -74,7 +141,10 @@ def test_read_gadget(tmp_path):
     assert info["npart"] == n ** 3 and info["skipped"] == 0 and info["a"] == pytest.approx(a)
     assert info["rhoc"] == pytest.approx(O.header_rhoc(4, n ** 3 // 4, pm * 1e10, 10.0e3), rel=1e-14)
     assert rel_l2(psi, psi_o) < 1e-12
+    # voutput.dat: the float vx grid after the velocity normalisation, in
+    # the reference layout (write_binary_vg, src/write3D.h:24-53)
     assert os.path.getsize(vout) == 4 * n ** 3
+    assert np.array_equal(np.fromfile(vout, np.float32), grids[1].astype(np.float32).ravel())
 
 
 def test_read_gadget_errors(tmp_path):
@@ -191,9 +261,9 @@ def test_cli_matches_library_and_restarts(tmp_path):
             t, _stop = trig(a, tn, 3, 7)
             if t is not None:
                 raw = np.fromfile(out + f"output{t}.dat", np.float64)
-                # the two runs deposit the IC with FP64 atomics in different orders
-                assert raw[2] == pytest.approx(a, rel=1e-13)
-                assert rel_l2(raw[3 + n:].view(np.complex128).reshape(n, n, n), s.get_psi()) < 1e-11
+                # the deposit fixes every cell's order: the two runs agree bit for bit
+                assert raw[2] == a
+                assert np.array_equal(raw[3 + n:].view(np.complex128).reshape(n, n, n), s.get_psi())
             st = s.step(*dt)
     # rerun from output4.dat (ti = 4) for the remaining steps
     out2 = str(tmp_path / "out2") + "/"
