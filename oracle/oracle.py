@@ -33,6 +33,12 @@ timestep_criteria      the kinetic / potential / self-interaction phase of
                        one step equals 2 pi in the da -> 0 limit (the
                        criterion's design); integrals vs closed-form
                        Einstein-de Sitter limit (Omega_m = 1)
+ic_from_particles      mass conservation, uniform lattice -> uniform rho,
+                       trilinear weights, periodic wrap, single velocity
+                       mode -> analytic theta; the float grids (f32) hold
+                       float values accumulated in particle order
+Triggers               write sequences worked out by hand from :238-284
+                       for a fixed a(tn) trajectory
 =====================  ===================================================
 """
 from __future__ import annotations
@@ -263,6 +269,67 @@ def simulate(rp: RunParams, psi0: np.ndarray, a0: float, nsteps: int, ref_maxphi
         used.append((dKE, dPE, dQI, a))
         st.split_op(dKE, dPE, dQI)                                     # :304
     return st, used
+
+
+class Triggers:
+    """The snapshot triggers of Simulation, src/CosmoBEC_MPI_stan.cpp:238-284
+    (the _stan chain), evaluated after Timestep_criteria has advanced a:
+    __call__ returns (snapshot number or None, break flag)."""
+
+    def __init__(self):
+        self.write_num, self.sten_counter, self.write_sten_counter = 0, 0, 2      # :208-210
+
+    def __call__(self, a: float, tn: int, twrite: int, timesteps: int):
+        t, stop = None, False
+        if a > 1:                                                      # :238
+            t, stop = 999999, True
+            self.write_num += 1
+        elif a > 0.714 and self.write_num == 4:                        # :244
+            t = 999998
+            self.write_num += 1
+        elif a > 0.5 and self.write_num == 2:                          # :250
+            t, stop = 999997, True
+            self.write_num += 1
+        elif a > 0.25 and self.write_num == 2:                         # :262
+            t = self.write_sten_counter
+            self.write_sten_counter += 1
+        elif a > 0.25 and self.write_num == 1:                         # :269
+            t = 999995
+            self.write_num += 1
+            self.sten_counter = 0
+        elif a > 0.105 and self.write_num == 0:                        # :276
+            t = 999994
+            self.write_num += 1
+        elif tn % twrite == 0 or tn == timesteps - 1:                  # :282
+            t = tn + 1
+        self.sten_counter += 1                                         # :287
+        return t, stop
+
+
+def run_main(rp: RunParams, psi0: np.ndarray, a0: float, twrite: int, timesteps: int, ti: int = 0):
+    """The reference's run from a fresh IC (main + Simulation,
+    src/CosmoBEC_MPI_stan.cpp:212-310): per step poisson (:218),
+    Timestep_criteria (:219), the snapshot triggers (:238-284; a snapshot
+    holds psi before the step and the advanced a, :219 then :276), then
+    Split_op_MPI (:304).  Returns [(t, a, psi), ...] for every snapshot
+    written, in order, and the number of steps run."""
+    st = State(rp, psi0)
+    a = a0
+    trig = Triggers()
+    snaps = []
+    steps = 0
+    for tn in range(ti, timesteps):
+        st.poisson()
+        m = st.maxima()
+        dKE, dPE, dQI, a, _ = timestep_criteria(rp, a, m["max_phi"], m["max_rho"])
+        t, stop = trig(a, tn, twrite, timesteps)
+        if t is not None:
+            snaps.append((t, a, st.psi.copy()))
+            if stop:
+                break
+        st.split_op(dKE, dPE, dQI)
+        steps += 1
+    return snaps, steps
 
 
 def ic_from_particles(rp: RunParams, chunks, box: float, pmass: float, a: float, ref_p1: bool = False,

@@ -1,5 +1,6 @@
 """Seeded synthetic wave functions for the parity tests (test helper)."""
 import math
+import os
 
 import numpy as np
 
@@ -47,3 +48,43 @@ def snapshot_bytes(psi: np.ndarray, xmax: float, a: float, mfdm_ev: float, g_ev_
 
 # inputs of the golden snapshot fixture (tests/golden/readsnap_n8_z3.txt)
 GOLDEN_SNAP = {"n": 8, "seed": 3, "xmax_mpc_h": 5.0, "a": 0.1, "mfdm_ev": 2.5e-24, "g_ev_cm3": 1e-60, "zslice": 3}
+
+
+def _upsample_axis(a: np.ndarray, n: int, axis: int) -> np.ndarray:
+    """Band-limited interpolation along one axis: the m-point spectrum
+    zero-padded to n points (the Nyquist term split between +m/2 and -m/2,
+    so the result passes through the m input points)."""
+    import scipy.fft as sf
+    m = a.shape[axis]
+    w = min(16, os.cpu_count() or 1)
+    A = sf.fft(a, axis=axis, workers=w)
+    shape = list(a.shape)
+    shape[axis] = n
+    B = np.zeros(shape, np.complex128)
+    lo = [slice(None)] * a.ndim
+    hi = [slice(None)] * a.ndim
+    lo[axis] = slice(0, m // 2)
+    hi[axis] = slice(n - (m // 2 - 1), n)
+    src_hi = [slice(None)] * a.ndim
+    src_hi[axis] = slice(m - (m // 2 - 1), m)
+    B[tuple(lo)] = A[tuple(lo)]
+    B[tuple(hi)] = A[tuple(src_hi)]
+    nyq_src = [slice(None)] * a.ndim
+    nyq_src[axis] = slice(m // 2, m // 2 + 1)
+    for dst in (m // 2, n - m // 2):
+        d = [slice(None)] * a.ndim
+        d[axis] = slice(dst, dst + 1)
+        B[tuple(d)] = 0.5 * A[tuple(nyq_src)]
+    del A
+    out = sf.ifft(B, axis=axis, workers=w, overwrite_x=True)
+    out *= n / m
+    return out
+
+
+def smooth_psi(n: int, m: int, seed: int, mfdm_solar: float, **kw) -> np.ndarray:
+    """make_psi at m^3, interpolated spectrally to n^3: a smooth field at
+    the large mesh sizes without an n^3 Gaussian random field."""
+    a = make_psi(m, seed, mfdm_solar, **kw)
+    for ax in range(3):
+        a = _upsample_axis(a, n, ax)
+    return a

@@ -18,7 +18,15 @@ long_128_oracle.npz  the 128^3 FP64 run of 1000 steps from the Zel'dovich
                      to the reference's end of run at a > 0.5, each split into
                      equal parts to make 1000 steps) and psi at steps 0, 100
                      and 1000 on 4096 seeded sample cells, with norm and
-                     maxima.  Takes a few minutes on 8 cores.
+                     maxima.  The IC is deposited the reference's way
+                     (float velocity grids, f32=True).  Takes a few minutes on
+                     8 cores.
+cli_64_oracle.npz    the reference's run with its ten parameters
+                     `2.5e-24 0 5 64 100 200 0 out/ ic 0` from the 64^3
+                     Zel'dovich IC (box 10 Mpc/h, z = 50, seed 1234) in the
+                     4-chunk sorted Gadget layout: every snapshot its trigger
+                     chain writes (number, a, psi on 4096 seeded cells, norm)
+                     and the SHA-256 of voutput.dat (the float vx grid).
 """
 import json
 import os
@@ -100,7 +108,7 @@ def long_fixture():
     n = L["n"]
     rp = O.RunParams(n=n, xmax_in=L["xmax_mpc_h"])
     pos, vel, pm, a0 = icgen.zeldovich(n, 2 * L["xmax_mpc_h"], z=50.0, seed=L["seed"])
-    psi0, _th, _sk = O.ic_from_particles(rp, [(pos, vel)], 2 * L["xmax_mpc_h"] * 1e3, pm * 1e10, a0)
+    psi0, _th, _sk = O.ic_from_particles(rp, [(pos, vel)], 2 * L["xmax_mpc_h"] * 1e3, pm * 1e10, a0, f32=True)
     steps, nad, a_end = long_schedule(rp, psi0, L["a_stop"], L["steps"])
     idx = np.random.default_rng(7).choice(n ** 3, L["samples"], replace=False)
     out = {"sched": np.array(steps), "adaptive_steps": nad, "a_end": a_end, "idx": idx, "a0": a0,
@@ -121,8 +129,32 @@ def long_fixture():
     np.savez_compressed(os.path.join(HERE, "long_128_oracle.npz"), **out)
 
 
+CLI64 = {"n": 64, "xmax_mpc_h": 5.0, "seed": 1234, "twrite": 100, "timesteps": 200, "samples": 1024}
+
+
+def cli64_fixture():
+    import hashlib
+    from ifdm_code_amd import icgen
+    L = CLI64
+    n = L["n"]
+    rp = O.RunParams(n=n, xmax_in=L["xmax_mpc_h"])
+    pos, vel, pm, a0 = icgen.zeldovich(n, 2 * L["xmax_mpc_h"], z=50.0, seed=L["seed"])
+    with tempfile.TemporaryDirectory() as d:
+        chunks = icgen.write_gadget(os.path.join(d, "ic"), pos, vel, pm, a0, 2 * L["xmax_mpc_h"] * 1e3)
+    psi0, _th, _sk, grids = O.ic_from_particles(rp, chunks, 2 * L["xmax_mpc_h"] * 1e3, pm * 1e10, a0, f32=True,
+                                                grids=True)
+    snaps, steps = O.run_main(rp, psi0, a0, L["twrite"], L["timesteps"])
+    idx = np.random.default_rng(9).choice(n ** 3, L["samples"], replace=False)
+    out = {"idx": idx, "a0": a0, "steps": steps, "t": np.array([t for t, _a, _p in snaps]),
+           "a": np.array([a for _t, a, _p in snaps]),
+           "psi": np.stack([p.ravel()[idx] for _t, _a, p in snaps]),
+           "norm": np.array([float(np.sum(np.abs(p) ** 2)) for _t, _a, p in snaps]),
+           "voutput_sha256": hashlib.sha256(grids[1].astype("<f4").tobytes()).hexdigest()}
+    np.savez_compressed(os.path.join(HERE, "cli_64_oracle.npz"), **out)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["readsnap", "integrals", "kdk", "long"]
+    which = sys.argv[1:] or ["readsnap", "integrals", "kdk", "long", "cli64"]
     if "readsnap" in which:
         readsnap_fixture()
     if "integrals" in which:
@@ -131,3 +163,5 @@ if __name__ == "__main__":
         kdk_fixture()
     if "long" in which:
         long_fixture()
+    if "cli64" in which:
+        cli64_fixture()

@@ -294,3 +294,62 @@ def test_snapshot_bytes_match_the_golden_layout(tmp_path):
     assert open(path, "rb").read() == snapshot_bytes(psi, cfg.xmax, g["a"], g["mfdm_ev"], g["g_ev_cm3"])
     assert (a, m, gg) == (g["a"], g["mfdm_ev"], g["g_ev_cm3"])
     assert np.array_equal(back, psi)
+
+
+def test_cli_reference_run_64(tmp_path):
+    """The reference's run with its ten parameters, `2.5e-24 0 5 64 100 200 0
+    out/ ic 0` (64^3 Zel'dovich IC, box 10 Mpc/h, z = 50, seed 1234, in the
+    4-chunk sorted Gadget layout): the driver runs the reference's trigger
+    chain (src/CosmoBEC_MPI_stan.cpp:238-284) to its end of run at a > 0.5
+    and must write the same snapshots as the oracle's run
+    (tests/golden/cli_64_oracle.npz): the same numbers, a to 1e-12, psi on
+    the fixture's sample cells to 1e-10, the norm to 1e-12, and voutput.dat
+    byte for byte."""
+    import hashlib
+    F, icgen, O = _mods()
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "cli_64_oracle.npz"))
+    n = 64
+    _say(f"[cli reference run] n={n}, {int(fx['steps'])} steps, {len(fx['t'])} snapshots")
+    exe = os.path.join(ROOT, "ifdm_code_amd", "ifdm")
+    pos, vel, pm, a0 = icgen.zeldovich(n, 10.0, z=50.0, seed=1234)
+    icgen.write_gadget(str(tmp_path / "ic" / "sorted"), pos, vel, pm, a0, 10.0e3)
+    out = str(tmp_path / "out") + "/"
+    os.makedirs(out)
+    args = [exe, "2.5e-24", "0", "5", str(n), "100", "200", "0", out, str(tmp_path / "ic" / "sorted"), "0"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    written = sorted(int(f[6:-4]) for f in os.listdir(out) if f.startswith("output") and f.endswith(".dat"))
+    assert written == sorted(int(t) for t in fx["t"])
+    idx = fx["idx"]
+    worst = 0.0
+    for k, t in enumerate(fx["t"]):
+        raw = np.fromfile(out + f"output{int(t)}.dat", np.float64)
+        assert raw.size == 3 + n + 2 * n ** 3
+        assert raw[2] == pytest.approx(float(fx["a"][k]), rel=1e-12)
+        psi = raw[3 + n:].view(np.complex128)
+        err = rel_l2(psi[idx], fx["psi"][k])
+        worst = max(worst, err)
+        assert err < 1e-10, (int(t), err)
+        assert abs(float(np.sum(np.abs(psi) ** 2)) / float(fx["norm"][k]) - 1) < 1e-12
+    _say(f"   worst sampled psi rel L2 over the snapshots: {worst:.3e}")
+    with open(out + "voutput.dat", "rb") as f:
+        assert hashlib.sha256(f.read()).hexdigest() == str(fx["voutput_sha256"])
+
+
+@pytest.mark.parametrize("order,mode,want", [("ABC", 0, 0.0), ("ACB", 0, 1.0 / 3), ("ABC", 1, 1.0 / 3),
+                                             ("ACB", 1, 1.0 / 3)])
+def test_deposit_float_rounding_in_particle_order(order, mode, want):
+    """Three particles on one cell centre with x-velocities 2^25, 1, -2^25
+    (tests/test_oracle_ic.py: float_order_case): the default deposit rounds
+    the float grid cell after each addition in the reference's particle
+    order (2^25, 1, -2^25 leaves 0; 2^25, -2^25, 1 leaves 1, then x 1/3 from
+    the normalisation); the FP64-atomic deposit gives 1/3 either way."""
+    from tests.test_oracle_ic import float_order_case
+    F, _icgen, _O = _mods()
+    pos, vel, box, pmass = float_order_case(order)
+    with F.Solver(F.Config.from_cli(2.5e-24, 0.0, 5.0, 8)) as s:
+        s.set_option("ic_deposit", mode)
+        _th, sk, vx = s.ic_from_particles([(pos, vel)], box, pmass, 1 / 51, want_vx=True)
+    assert sk == 0
+    assert vx[3, 3, 3] == np.float32(want)
+    assert np.count_nonzero(vx) == (1 if want else 0)

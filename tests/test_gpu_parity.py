@@ -380,10 +380,14 @@ def test_long_run_against_oracle_fixture():
 
 def test_mixed_long_run_density_error():
     """The mixed path over the 1000-step 128^3 schedule of the oracle
-    fixture (g = 0) against the FP64 path from the same IC.  The drift's
+    fixture (box 10 Mpc/h, g = 0) against the FP64 path from the same IC,
+    beside an FP64 run from the IC rounded to complex64 (the error that
+    rounding alone grows into: the dynamics' own sensitivity).  The drift's
     transforms on psi use accurate twiddles; with FP32 twiddles their fixed
-    errors compound over the steps.  Measured density rel L2 after 1000
-    steps: 2.4e-4 (FP32 twiddles: 1.0e-2); the bound separates the two."""
+    errors compound over the steps (1.0e-2 after 1000 steps).  Measured
+    density rel. L2 after 1000 steps: 2.4e-4, which is this box's limit
+    for FP32 arithmetic, not the 1e-4 bar (met on the 512^3 headline
+    configuration, tests/test_gpu_large.py); the bound is 3e-4."""
     import os
     from ifdm_code_amd import icgen
     F, _O = _mods()
@@ -395,25 +399,28 @@ def test_mixed_long_run_density_error():
     with F.Solver(_cfg(F, n)) as s:
         s.ic_from_particles([(pos, vel)], 10.0e3, pm * 1e10, a0)
         psi0 = s.get_psi()
+    e = int(np.frexp(np.max(np.abs(psi0)))[1])
+    psi0_c64 = (psi0 * 2.0 ** -e).astype(np.complex64).astype(np.complex128) * 2.0 ** e
     rho = {}
-    for prec in (F.FP64, F.MIXED):
+    for key, prec, p0 in (("fp64", F.FP64, psi0), ("mixed", F.MIXED, psi0), ("rounded_ic", F.FP64, psi0_c64)):
         with F.Solver(F.Config.from_cli(2.5e-24, 0.0, 5.0, n, precision=prec)) as s:
-            s.set_psi(psi0)
+            s.set_psi(p0)
             s.poisson()
             s.run(sched)
-            rho[prec] = np.abs(s.get_psi()) ** 2
-    err = rel_l2(rho[F.MIXED], rho[F.FP64])
-    _say(f"   mixed density rel L2 after {len(sched)} steps = {err:.3e}")
-    assert err < 1e-3
+            rho[key] = np.abs(s.get_psi()) ** 2
+    err = rel_l2(rho["mixed"], rho["fp64"])
+    floor = rel_l2(rho["rounded_ic"], rho["fp64"])
+    _say(f"   mixed density rel L2 after {len(sched)} steps = {err:.3e} "
+         f"(FP64 from the complex64-rounded IC: {floor:.3e})")
+    assert err < 3e-4
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_plane_wave_drift_1024(precision):
-    """The largest supported mesh, 1024^3, in both precisions: a plane wave
-    psi = A exp(i k.x) only changes phase under the drift, by
+@pytest.mark.parametrize("n,precision", [(512, 0), (512, 1), (1024, 0), (1024, 1)])
+def test_plane_wave_drift_large(n, precision):
+    """The production meshes, 512^3 and 1024^3, in both precisions: a plane
+    wave psi = A exp(i k.x) only changes phase under the drift, by
     exp(-i hbar |k|^2 dKE / (2 m)) (src/struct3D.h:121), uniformly."""
     import math
     F, O = _mods()
-    n = 1024
     _say(f"[plane wave] n={n} precision={precision}")
     rp = _rp(O, n)
     cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n, precision=precision)
@@ -437,15 +444,15 @@ def test_plane_wave_drift_1024(precision):
     assert err < (1e-11 if precision == 0 else 2e-5)
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_poisson_cosine_1024(precision):
-    """The Poisson solve (source, the half-spectrum passes, c2r) at 1024^3:
+@pytest.mark.parametrize("n,precision", [(512, 0), (512, 1), (1024, 0), (1024, 1)])
+def test_poisson_cosine_large(n, precision):
+    """The Poisson solve (source, the half-spectrum passes, c2r) at 512^3 and 1024^3:
     a density with one Fourier mode, |psi|^2 = A^2 (1 + 0.1 cos k.x), has
     the potential C cos k.x for the same physical k at any mesh size
     (-4 pi G / |k|^2 on the continuous k, src/CosmoBEC_MPI_stan.cpp:104-110;
     the constant part of the source is the k = 0 mode, set to zero).  C is
     taken from the FP64 solve at 32^3 (pinned against the oracle by
-    test_poisson_matches_oracle); the 1024^3 potential must be C cos k.x."""
+    test_poisson_matches_oracle); the large mesh's potential must be C cos k.x."""
     import math
     F, O = _mods()
     m = (3, -5, 7)
@@ -474,7 +481,6 @@ def test_poisson_cosine_1024(precision):
     c32 = (ex[:, None, None] * ey[None, :, None] * ez[None, None, :]).real
     C = float(np.sum(phi32 * c32) / np.sum(c32 * c32))
     assert rel_l2(phi32, C * c32) < 1e-13              # one mode at 32^3 already
-    n = 1024
     _say(f"[poisson cosine] n={n} precision={precision} C={C:.6e}")
     phi, st, (ex, ey, ez) = solve(n, precision, 64)
     err = 0.0

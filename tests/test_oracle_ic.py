@@ -98,6 +98,37 @@ def test_reference_float_velocity_grids_differ_at_float_roundoff():
     assert 1e-9 < d < 1e-5
 
 
+def float_order_case(order):
+    """Three particles on the centre of cell (3, 3, 3) of an 8^3 mesh with
+    cell size 1 kpc/h (weight 1 there, 0 elsewhere, Lg^3 = 1): x-velocities
+    2^25, 1 and -2^25 visited in `order`.  Returns (pos, vel, box, pmass)."""
+    v = {"A": 2.0 ** 25, "B": 1.0, "C": -(2.0 ** 25)}
+    pos = np.full((3, 3), 3.5, np.float32)
+    vel = np.zeros((3, 3), np.float32)
+    vel[:, 0] = [v[c] for c in order]
+    return pos, vel, 8.0, 2.0
+
+
+@pytest.mark.parametrize("order,f32,want", [("ABC", True, 0.0), ("ACB", True, 1.0 / 3), ("ABC", False, 1.0 / 3),
+                                            ("ACB", False, 1.0 / 3)])
+def test_float_grids_accumulate_in_particle_order(order, f32, want):
+    """The reference adds each contribution into a float grid cell
+    (vxg[c] += ..., std::vector<float>, src/gadget_IC.h:112-150,583): in
+    float, 2^25 + 1 rounds back to 2^25 (the float spacing there is 4), so
+    visiting 2^25, 1, -2^25 leaves 0 where 2^25, -2^25, 1 leaves 1; FP64
+    grids give 1 either way.  The normalisation then multiplies by
+    pmass / rho = 1/3 (:645-651) and rounds to float."""
+    pos, vel, box, pmass = float_order_case(order)
+    rp = O.RunParams(n=8, xmax_in=5.0)
+    _psi, _th, sk, (rho, vx, vy, vz) = O.ic_from_particles(rp, [(pos, vel)], box, pmass, 1 / 51, f32=f32,
+                                                           grids=True)
+    assert sk == 0
+    assert rho[3, 3, 3] == 3 * pmass and np.count_nonzero(rho) == 1
+    expect = float(np.float32(want)) if f32 else want
+    assert vx[3, 3, 3] == expect
+    assert np.count_nonzero(vx) == (1 if want else 0)
+
+
 def test_ref_p1_ghosts_double_deposit():
     """At one rank the reference deposits the particles beyond the last cell
     centre twice (ghost copy + owner)."""

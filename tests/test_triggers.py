@@ -55,3 +55,21 @@ def test_a_beyond_one_stops():
     import ifdm_code_amd as F
     trig = F.OutputTriggers(0, True)
     assert trig(1.2, 5, 100, 1000) == (999999, True)
+
+
+def test_oracle_trigger_chain_matches():
+    """The oracle's restatement of the trigger chain (oracle.Triggers, used
+    for the 64^3 driver fixture) gives the same hand-worked sequence."""
+    from oracle import oracle as O
+    t = O.Triggers()
+    out, stop_tn = [], None
+    for tn in range(1000):
+        a = 0.021 + 0.03 * (tn + 1)
+        w, stop = t(a, tn, 100, 1000)
+        if w is not None:
+            out.append((tn, w))
+        if stop:
+            stop_tn = tn
+            break
+    expect, lib_stop = _run(0)
+    assert out == expect and stop_tn == lib_stop
