@@ -80,7 +80,7 @@ class _ICInfo(ctypes.Structure):
 
 
 EXPORTS = [
-    "ifdm_power_spectrum",
+    "ifdm_power_spectrum", "ifdm_power_spectrum_noalloc",
     "ifdm_run_checkpointed", "ifdm_adjoint", "ifdm_adjoint_device", "ifdm_set_psi_device", "ifdm_get_psi_device",
     "ifdm_read_gadget", "ifdm_ic_begin", "ifdm_ic_deposit", "ifdm_ic_finish", "ifdm_set_rhoc",
     "ifdm_write_snapshot", "ifdm_read_snapshot",
@@ -119,6 +119,7 @@ def lib():
     L.ifdm_time_passes.argtypes = [vp, ctypes.POINTER(_Dt), ip, ctypes.POINTER(ctypes.c_float)]
     L.ifdm_fft3d.argtypes = [vp, ip]
     L.ifdm_power_spectrum.argtypes = [vp, ip, dp, dp, dp]
+    L.ifdm_power_spectrum_noalloc.argtypes = [vp, ip, dp, dp, dp]
     L.ifdm_run_checkpointed.argtypes = [vp, ctypes.POINTER(_Dt), ip, ip]
     L.ifdm_adjoint.argtypes = [vp, ctypes.POINTER(_Dt), ip, dp]
     L.ifdm_adjoint_device.argtypes = [vp, ctypes.POINTER(_Dt), ip, vp]
@@ -262,11 +263,14 @@ class Solver:
         self._chk(lib().ifdm_set_phi(self._h, _dp(a)), "ifdm_set_phi")
 
     # ---- diagnostics
-    def power_spectrum(self, nbins: int | None = None):
-        """(k [1/Mpc], P(k) [Mpc^3], modes) on shells of width 2 pi / L."""
+    def power_spectrum(self, nbins: int | None = None, noalloc: bool = False):
+        """(k [1/Mpc], P(k) [Mpc^3], modes) on shells of width 2 pi / L.
+        noalloc: the entry point that allocates no device memory
+        (ifdm_power_spectrum_noalloc, at most 4096 bins)."""
         nb = nbins or self.cfg.n // 2
         k, pk, nm = np.zeros(nb), np.zeros(nb), np.zeros(nb)
-        self._chk(lib().ifdm_power_spectrum(self._h, nb, _dp(k), _dp(pk), _dp(nm)), "ifdm_power_spectrum")
+        fn = "ifdm_power_spectrum_noalloc" if noalloc else "ifdm_power_spectrum"
+        self._chk(getattr(lib(), fn)(self._h, nb, _dp(k), _dp(pk), _dp(nm)), fn)
         return k, pk, nm
 
     # ---- adjoint

@@ -31,6 +31,7 @@ struct ifdm_ctx {
     double *red_stage = nullptr;  // 3 x IFDM_RED_BLOCKS doubles
     void *dtab = nullptr;      // N complex: per-step drift table
     double *red_host = nullptr;  // 3 doubles, pinned
+    double *pk_scratch = nullptr;  // device address of this device's static P(k) scratch
     int nz_blocks = 0;
     int rho_parts = 0;   // density partials written by the last source pass
     // physics

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,32 @@
 using namespace ifdm;
 
 static thread_local std::string g_create_err;
+
+// Scratch of ifdm_power_spectrum_noalloc: the binned sums (3 per bin) and
+// the normalised output (3 per bin), at most IFDM_PK_MAX_BINS bins.  Each
+// GPU holds its own copy for the life of the process; every call on a GPU
+// shares it (calls that overlap on one GPU race).  Host code uses the
+// device address from hipGetSymbolAddress, never the array's name.
+#define IFDM_PK_MAX_BINS 4096
+static __device__ __align__(256) double scratch[6 * IFDM_PK_MAX_BINS];
+static constexpr int IFDM_MAX_DEVICES = 64;
+static double *g_scratch_ptr[IFDM_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// the device address of `scratch` on the current device `dev`, resolved
+// once per device
+static hipError_t scratch_address(int dev, double **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= IFDM_MAX_DEVICES) return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[dev]) {
+        double *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
 
 namespace {
 
@@ -230,14 +257,21 @@ struct Ops {
     }
 
     // forward 3-D real transform of m_s|psi|^2 - rho_c into shat, binned
-    static int power_spectrum(ifdm_ctx *c, int nbins, double *dev_out) {
+    // into dev_out (3 nbins doubles), cleared by a fill kernel when
+    // fill_clear (no memset node for callers that capture the stream)
+    static int power_spectrum(ifdm_ctx *c, int nbins, double *dev_out, bool fill_clear) {
         const SArgs sa = sargs(c, 0.0);
         int rc;
         c->rho_parts = c->nz_blocks;
         if ((rc = zpass<Z_SRC>(c, zargs(c, 0, 0), c->part_rho))) return rc;
         if ((rc = spass<1, S_FWD>(c, c->shat, c->hs, c->sps, N / 2 + 1, sa))) return rc;
         if ((rc = spass<0, S_FWD>(c, c->shat, c->hs, c->sps, N / 2 + 1, sa))) return rc;
-        HIPCHK(c, hipMemsetAsync(dev_out, 0, sizeof(double) * 3 * nbins, c->stream));
+        if (fill_clear) {
+            hipLaunchKernelGGL(k_fill0, dim3((3 * nbins + 255) / 256), dim3(256), 0, c->stream, dev_out, 3 * nbins);
+            LAUNCHCHK(c);
+        } else {
+            HIPCHK(c, hipMemsetAsync(dev_out, 0, sizeof(double) * 3 * nbins, c->stream));
+        }
         const long nh = (long)N * N * (N / 2 + 1);
         const unsigned nb = (unsigned)std::min<long>((nh + 255) / 256, 2048);
         hipLaunchKernelGGL((k_pk_bin<T, N>), dim3(nb), dim3(256), sizeof(double) * 3 * nbins, c->stream,
@@ -313,7 +347,7 @@ struct Ops {
 };
 
 struct Dispatch {
-    int (*power_spectrum)(ifdm_ctx *, int, double *);
+    int (*power_spectrum)(ifdm_ctx *, int, double *, bool);
     int (*adj_kick)(ifdm_ctx *, double, double);
     int (*theta_solve)(ifdm_ctx *, double);
     int (*pass)(ifdm_ctx *, int, double, double, double);
@@ -491,6 +525,8 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
         return fail(IFDM_E_HIP, std::string("hipDeviceGetAttribute: ") + hipGetErrorString(e));
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
         return fail(IFDM_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    if ((e = scratch_address(cfg->device, &c->pk_scratch)) != hipSuccess)
+        return fail(IFDM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
     // element sizes of the stored fields (reductions stay FP64)
     const size_t rs = c->mixed ? sizeof(float) : sizeof(double), cs = 2 * rs, ds = sizeof(double);
     // z-pass block count: lines per block = min(256 / (n / E), n * n)
@@ -808,7 +844,7 @@ int ifdm_power_spectrum(ifdm_ctx *c, int nbins, double *k_mean, double *pk, doub
     get_dispatch(c->n, c->cfg.precision, &d);
     double *dev = nullptr;
     HIPCHK(c, hipMalloc((void **)&dev, sizeof(double) * 3 * nbins));
-    int rc = d.power_spectrum(c, nbins, dev);
+    int rc = d.power_spectrum(c, nbins, dev, false);
     std::vector<double> h(3 * (size_t)nbins);
     ifdm_stats st;
     if (!rc) rc = read_stats(c, &st);
@@ -830,6 +866,32 @@ int ifdm_power_spectrum(ifdm_ctx *c, int nbins, double *k_mean, double *pk, doub
         if (nmodes) nmodes[b] = w;
     }
     return IFDM_OK;
+}
+
+int ifdm_power_spectrum_noalloc(ifdm_ctx *c, int nbins, double *k_mean, double *pk, double *nmodes) {
+    if (!c || !pk) return IFDM_E_ARG;
+    if (nbins <= 0 || nbins > IFDM_PK_MAX_BINS) {
+        c->err = "ifdm_power_spectrum_noalloc: nbins must be in 1.." + std::to_string(IFDM_PK_MAX_BINS);
+        return IFDM_E_ARG;
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    Dispatch d;
+    get_dispatch(c->n, c->cfg.precision, &d);
+    double *sums = c->pk_scratch, *outp = c->pk_scratch + 3 * IFDM_PK_MAX_BINS;
+    int rc = d.power_spectrum(c, nbins, sums, true);
+    c->ev_pass.clear();
+    if (rc) return rc;
+    const double L = 2.0 * c->cfg.xmax;
+    hipLaunchKernelGGL(k_pk_finalize, dim3((nbins + 255) / 256), dim3(256), 0, c->stream, sums, nbins, c->red,
+                       c->ms * c->psc2, (double)c->n3, std::pow(L, 3), std::pow((double)c->n, 6), 2.0 * M_PI / L,
+                       outp);
+    LAUNCHCHK(c);
+    const size_t nb = sizeof(double) * (size_t)nbins;
+    HIPCHK(c, hipMemcpyAsync(pk, outp, nb, hipMemcpyDeviceToHost, c->stream));
+    if (k_mean) HIPCHK(c, hipMemcpyAsync(k_mean, outp + nbins, nb, hipMemcpyDeviceToHost, c->stream));
+    if (nmodes) HIPCHK(c, hipMemcpyAsync(nmodes, outp + 2 * nbins, nb, hipMemcpyDeviceToHost, c->stream));
+    // the reductions of this pass, as ifdm_power_spectrum leaves them
+    return read_stats(c, nullptr);
 }
 
 // ---------------------------------------------------------------- adjoint

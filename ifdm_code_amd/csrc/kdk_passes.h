@@ -723,6 +723,29 @@ __global__ __launch_bounds__(256) void k_pk_bin(const cx<T> *__restrict__ shat, 
         if (sb[i] != 0.0) atomicAdd(&out[i], sb[i]);
 }
 
+// zero fill (used where a caller may capture the stream in a graph, in
+// place of a memset node)
+__global__ __launch_bounds__(256) void k_fill0(double *__restrict__ p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// P(k) from the binned sums (sb[3b] = sum w |s_k|^2, sb[3b+1] = sum w |n|,
+// sb[3b+2] = sum w) and the device reductions red = {max|phi|, max|psi|^2,
+// sum|psi|^2} of the same pass: out[b] = L^3 sum / (rho_mean^2 N^6 w),
+// out[nb + b] = kf sum|n| / w, out[2 nb + b] = w, with rho_mean =
+// m_s psc2 sum|psi|^2 / N^3 (psc2: the mixed mode's psi scale squared)
+__global__ __launch_bounds__(256) void k_pk_finalize(const double *__restrict__ sb, int nb,
+                                                     const double *__restrict__ red, double ms_psc2, double n3,
+                                                     double l3, double n6, double kf, double *__restrict__ out) {
+    const double rho_mean = ms_psc2 * red[2] / n3;
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+        const double w = sb[3 * b + 2];
+        out[b] = w > 0 ? l3 * sb[3 * b] / (rho_mean * rho_mean * n6) / w : 0.0;
+        out[nb + b] = w > 0 ? kf * sb[3 * b + 1] / w : kf * (b + 1);
+        out[2 * nb + b] = w;
+    }
+}
+
 // plain 16-byte-vector copy: the achievable streaming bandwidth reference
 __global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long n) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)

@@ -191,6 +191,16 @@ int ifdm_fft3d(ifdm_ctx *ctx, int sign);
  * nmodes may be NULL.  Uses the Poisson scratch; psi and phi are kept. */
 int ifdm_power_spectrum(ifdm_ctx *ctx, int nbins, double *k_mean, double *pk, double *nmodes);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight diagnostic loops, code that must not call hipMalloc): the
+ * same P(k) as ifdm_power_spectrum, with the same outputs, but it allocates
+ * nothing.  Its bins live in a fixed scratch array of the library on each
+ * GPU (6 x 4096 doubles, 192 KiB, held until the process ends), cleared by
+ * a kernel on every call; nbins is at most 4096 (IFDM_E_ARG beyond).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (two
+ * contexts on the same device, two threads) race. */
+int ifdm_power_spectrum_noalloc(ifdm_ctx *ctx, int nbins, double *k_mean, double *pk, double *nmodes);
+
 /* ---- adjoint (no counterpart in the reference) ----------------------- */
 
 /* Forward run of nsteps steps with the given schedule (as ifdm_run) that
