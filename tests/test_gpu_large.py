@@ -64,16 +64,24 @@ def test_plane_wave_kdk_steps_closed_form(n, prec):
     with F.Solver(cfg) as s:
         s.set_psi(psi0)
         st = s.poisson()
-        assert st["max_phi"] < 1e-12 * abs(rp.bigG * rp.rhoc * (2 * rp.xmax) ** 2)
+        # the potential of a 1 % box-scale density mode is ~ G rho L^2 / 100
+        assert st["max_phi"] < (1e-12 if prec == 0 else 1e-6) * O.bigG * rp.rhoc * (2 * rp.xmax) ** 2
         s.run(sched)
         out = s.get_psi()
         st = s.poisson()
     want = complex(math.cos(phase), -math.sin(phase))
-    err = 0.0
+    err, ss = 0.0, 0.0
     for i in range(0, n, 64):                     # chunks keep the temporaries small
-        err = max(err, float(np.max(np.abs(out[i:i + 64] / psi0[i:i + 64] - want))))
-    _say(f"   max |psi_3 / psi_0 - exp(-i {phase:.3f})| = {err:.3e}; norm drift {st['norm'] / (amp * amp * n ** 3) - 1:.2e}")
-    assert err < (1e-11 if prec == 0 else 2e-5)
+        d = np.abs(out[i:i + 64] / psi0[i:i + 64] - want)
+        err = max(err, float(np.max(d)))
+        ss += float(np.sum(d * d))
+    rms = math.sqrt(ss / n ** 3)
+    _say(f"   |psi_3 / psi_0 - exp(-i {phase:.3f})|: max {err:.3e}, rms {rms:.3e}; "
+         f"norm drift {st['norm'] / (amp * amp * n ** 3) - 1:.2e}")
+    # complex floats: FP32 round-off of 18 transforms per cell (1024^3 keeps
+    # FP32 twiddles); the maximum is over up to 2^30 cells
+    assert err < (1e-11 if prec == 0 else 1e-4)
+    assert rms < (1e-12 if prec == 0 else 1e-5)
     assert abs(st["norm"] / (amp * amp * n ** 3) - 1) < (1e-12 if prec == 0 else 1e-6)
 
 
