@@ -76,6 +76,7 @@ struct Ops {
         s.dKE = dKE;
         s.green = c->green;
         s.fftnorm = c->fftnorm;
+        s.kunit2 = std::pow(M_PI / c->cfg.xmax, 2);
         s.rev = 0;
         s.aphi = 0.0;
         s.arho = 0.0;
@@ -130,13 +131,15 @@ struct Ops {
         int rc;
         if ((rc = prof_begin(c))) return rc;
         bool done = false;
-        // (persistent work-groups spill registers at N = 1024 in FP64: up to N = 512)
+        // (persistent work-groups spill registers at N = 1024: up to N = 512;
+        // complex floats at 1024 need 190-238 VGPRs, or spill 260-412 bytes per
+        // lane under the 1024-thread cap of 128)
         if constexpr (MODE != S_KICK_DRIFT1 && N <= 512) {
           constexpr bool PSI = MODE == S_DRIFT1;
           constexpr int bit = PSI ? (AXIS == 1 ? 1 : 2) : (AXIS == 0 ? 8 : (MODE == S_INV ? 16 : 4));
           if (c->persistent & bit) {
             done = true;
-            constexpr int TC = s_cols<T, N, false, MODE>();
+            constexpr int TC = s_cols<T, N, false, MODE, AXIS>();
             const int ntz = (kv + TC - 1) / TC;
             static int occ = -1;   // resident work-groups per CU for this instantiation
             if (occ < 0) {
@@ -153,8 +156,15 @@ struct Ops {
           }
         }
         if (!done) {
-            constexpr int TC = s_cols<T, N, WIDE, MODE>();
-            const int ntz = (kv + TC - 1) / TC;
+            constexpr int TC = s_cols<T, N, WIDE, MODE, AXIS>();
+            int ntz = (kv + TC - 1) / TC;
+            if constexpr (s_narrow<T, N, WIDE, AXIS, MODE>()) {
+                // a tile count rounded up to the XCD group (the extra tiles
+                // exit at once), and a grid of a multiple of 8 groups
+                constexpr int G = xcd_group<MODE>();
+                ntz = (ntz + G - 1) / G * G;
+                if (no % 8) { c->err = "narrow strided pass: line count must be a multiple of 8"; return IFDM_E_ARG; }
+            }
             hipLaunchKernelGGL((k_spass<T, N, AXIS, MODE, WIDE>), dim3(ntz, no), dim3(TC * P), 0, c->stream,
                                (cx<T> *)arr, rl, ps, kv, (const cx<T> *)c->tw, (const cx<double> *)c->tw64, sa);
         }

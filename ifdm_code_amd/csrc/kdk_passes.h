@@ -57,6 +57,7 @@ struct SArgs {
     double dKE;        // time integral of the drift
     double green;      // -4 pi G
     double fftnorm;    // 1 / N^3
+    double kunit2;     // (2 pi / L)^2: k_i^2 = kunit2 m_i^2, m_i the signed mode index
     int rev;           // blocks walk the fixed axis in descending order
     // the kick of S_KICK_DRIFT1: phase aphi phi + arho |psi|^2, phi a real
     // field of psi's shape
@@ -101,16 +102,30 @@ template <typename T, int N> constexpr int z_lines() {
     constexpr int P = N / SchedT<T, N>::E;
     return (256 / P) < (N * N) ? (256 / P) : (N * N);
 }
+// N = 1024 y passes (AXIS 1): 64-byte row segments ("narrow" tiles), so a
+// tile's exchange buffer takes 74 KB of LDS and two work-groups share a CU,
+// one loading while the other transforms (a 128-byte tile of 1024-point
+// lines needs 149 KB: one work-group per CU, whose loads and transforms
+// never overlap).  The two tiles of a 128-byte line run on one XCD at about
+// the same time (xcd_pair_rows), so the second finds the line in L2.  The
+// x passes (AXIS 0) keep 128-byte rows: their rows lie in N different
+// x-planes, one address translation each, and narrow rows would move half
+// the bytes per translation (measured at 1024^3: B 5.8 -> 6.9 ms mixed,
+// 9.9 -> 11.3 ms FP64).  Measured at 1024^3 with the pairing (without):
+// A 8.03 -> 7.59 (8.51) ms mixed, 13.09 -> 12.34 (14.07) ms FP64; the
+// half-spectrum y passes F, H 1.88, 1.91 (1.97, 2.01) ms mixed; in FP64 they
+// keep 128-byte rows (3.57, 3.46 against 3.64, 3.65 ms narrow).
+template <typename T, int N, bool WIDE = false, int AXIS = 1, int MODE = -1> constexpr bool s_narrow() {
+    return N == 1024 && !WIDE && AXIS == 1 && (sizeof(T) == 4 || MODE == S_KICK_DRIFT1 || MODE == S_DRIFT1);
+}
+
 // z-columns per strided-pass tile: 128-byte row segments (8 complex
-// doubles, 16 complex floats); WIDE doubles them for FP64 (256-byte rows,
-// used by the 3-D transform's x pass), and a work-group never exceeds 1024
-// threads.
-// Complex floats at N = 1024 take 64-byte rows on the half spectrum's y
-// passes F and H, so two work-groups fit a CU's LDS (measured at 1024^3:
-// F 2.50 -> 1.95 ms, H 2.32 -> 1.90 ms; passes A, B and G lose with them).
-template <typename T, int N, bool WIDE = false, int MODE = -1> constexpr int s_cols() {
-    constexpr bool narrow = sizeof(T) == 4 && N == 1024 && !WIDE && (MODE == S_FWD || MODE == S_INV);
-    constexpr int base = narrow ? 8 : 128 / (int)sizeof(cx<T>);
+// doubles, 16 complex floats; 64-byte ones for narrow tiles); WIDE doubles
+// them for FP64 (256-byte rows, used by the 3-D transform's x pass), and a
+// work-group never exceeds 1024 threads.
+template <typename T, int N, bool WIDE = false, int MODE = -1, int AXIS = 1> constexpr int s_cols() {
+    constexpr bool narrow = s_narrow<T, N, WIDE, AXIS, MODE>();
+    constexpr int base = narrow ? 64 / (int)sizeof(cx<T>) : 128 / (int)sizeof(cx<T>);
     constexpr int want = (WIDE && sizeof(cx<T>) == 16) ? 2 * base : base;
     constexpr int cap = 1024 / (N / SchedT<T, N>::E);
     // and the exchange buffer within the 160 KB of LDS
@@ -421,17 +436,22 @@ __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const T
 #pragma unroll
             for (int e = 0; e < E; ++e) v[e] = cmul(v[e], tab[t + e * P]);
         } else {
-            const double k2o = k2[other], k2k = valid ? k2[k] : 0.0;
             if constexpr (sizeof(T) == 4) {
                 // complex floats: the Green factor in FP32 (an approximate
                 // reciprocal, ~1e-7; the potential is recomputed each step)
-                const float g = (float)(sa.green * sa.fftnorm);
+                // from the integer |m|^2 = m_x^2 + m_y^2 + m_z^2 (exact), no
+                // table loads: k^2 = (2 pi / L)^2 |m|^2
+                const float g = (float)(sa.green * sa.fftnorm / sa.kunit2);
+                const int mo = other < N / 2 ? other : other - N;
+                const int mok = mo * mo + (valid ? k * k : 0);     // k <= N/2 on the half spectrum
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
-                    const double kk = k2[t + e * P] + k2o + k2k;
-                    v[e] = cscale(v[e], kk == 0.0 ? 0.f : g * __builtin_amdgcn_rcpf((float)kk));
+                    const int i = t + e * P, mi = i < N / 2 ? i : i - N;
+                    const int m2 = mi * mi + mok;
+                    v[e] = cscale(v[e], m2 == 0 ? 0.f : g * __builtin_amdgcn_rcpf((float)m2));
                 }
             } else {
+                const double k2o = k2[other], k2k = valid ? k2[k] : 0.0;
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     const double kk = k2[t + e * P] + k2o + k2k;
@@ -505,22 +525,47 @@ __device__ __forceinline__ void xcd_pair(int &bx, int &by) {
 
 // Minimum waves per SIMD asked of the compiler for a strided pass: 8 caps
 // the complex-float Green pass at 64 VGPRs, so two 1024-thread work-groups
-// share a CU (measured at 512^3: 0.39 -> 0.35 ms, one 8-byte spill), for
-// the sizes where two such work-groups fit the LDS; the other passes fit
+// share a CU (measured at 512^3: 0.39 -> 0.35 ms), for the sizes where two
+// such work-groups fit the LDS; 4 caps the narrow 512-thread tiles of
+// N = 1024 at 128 VGPRs, so two of them share a CU; the other passes fit
 // without a cap (capping the z-pass C spilled and slowed).
-template <typename T, int MODE, int N> constexpr int sp_wpe() {
-    return (MODE == S_GREEN && sizeof(T) == 4 && N >= 128 && N <= 512) ? 8 : 1;
+template <typename T, int MODE, int N, bool WIDE = false, int AXIS = 1> constexpr int sp_wpe() {
+    return (MODE == S_GREEN && sizeof(T) == 4 && N >= 128 && N <= 512) ? 8
+                                                                          : (s_narrow<T, N, WIDE, AXIS, MODE>() ? 4 : 1);
 }
+
+// Narrow tiles: the grid is (a multiple of G column tiles, lines of the
+// fixed axis).  Within each run of 8G linear ids, id 8Gg + 8h + x takes the
+// virtual tile 8Gg + Gx + h (row-major over the grid), so the G column
+// tiles Gm .. Gm + G - 1 of a row start 8 ids apart, on one XCD (ids go
+// round-robin over the 8 XCDs): G = 2 for the two 64-byte halves of a
+// 128-byte psi segment, G = 4 where four tiles also share a 128-byte phi
+// segment (the kick pass: 32-byte phi rows).  Needs gridDim.x * gridDim.y
+// to be a multiple of 8G.
+template <int G> __device__ __forceinline__ void xcd_group_rows(int &bx, int &by) {
+    const int L = blockIdx.x + gridDim.x * blockIdx.y;
+    const int r = L & (8 * G - 1);
+    const int V = (L & ~(8 * G - 1)) + G * (r & 7) + (r >> 3);
+    by = V / (int)gridDim.x;
+    bx = V - by * (int)gridDim.x;
+}
+// tiles per XCD group of a narrow pass
+template <int MODE> constexpr int xcd_group() { return MODE == S_KICK_DRIFT1 ? 4 : 2; }
 
 // one tile per work-group: grid (z-column tiles, lines of the fixed axis)
 template <typename T, int N, int AXIS, int MODE, bool WIDE>
-__global__ __launch_bounds__((s_cols<T, N, WIDE, MODE>() * (N / SchedT<T, N>::E)), (sp_wpe<T, MODE, N>()))
+__global__ __launch_bounds__((s_cols<T, N, WIDE, MODE, AXIS>() * (N / SchedT<T, N>::E)), (sp_wpe<T, MODE, N, WIDE, AXIS>()))
 void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa) {
-    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE, MODE>(), LS = lds_col_line<N>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE, MODE, AXIS>(), LS = lds_col_line<N>();
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     int bx = blockIdx.x, by = blockIdx.y;
-    if constexpr (MODE == S_KICK_DRIFT1 && (N / TC) % 16 == 0) xcd_pair(bx, by);
+    if constexpr (s_narrow<T, N, WIDE, AXIS, MODE>()) {
+        xcd_group_rows<xcd_group<MODE>()>(bx, by);
+        if (bx * TC >= kv) return;            // a padding tile of the rounded-up count (whole work-group)
+    } else if constexpr (MODE == S_KICK_DRIFT1 && (N / TC) % 16 == 0) {
+        xcd_pair(bx, by);
+    }
     const int k0 = bx * TC, k = k0 + c;
     const int other = sa.other0 + block_order(sa.rev, by, gridDim.y);
     const bool valid = k < kv;
@@ -559,10 +604,10 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
 // overlap the butterflies and LDS exchanges.  The first LDS write of every
 // transform is preceded by a barrier, so consecutive tiles never race on LDS.
 template <typename T, int N, int AXIS, int MODE>
-__global__ __launch_bounds__((s_cols<T, N, false, MODE>() * (N / SchedT<T, N>::E)))
+__global__ __launch_bounds__((s_cols<T, N, false, MODE, AXIS>() * (N / SchedT<T, N>::E)))
 void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa, int ntz,
                 int ntiles) {
-    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, false, MODE>(), LS = lds_col_line<N>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, false, MODE, AXIS>(), LS = lds_col_line<N>();
     constexpr int NT = TC * P;
     __shared__ cx<T> lds[TC * LS];
     // the spectral multiply's table in LDS, so no global load is issued
