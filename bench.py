@@ -86,7 +86,26 @@ def cpu_baseline(n: int, psi: np.ndarray, cfg_g_in: float, xmax_in: float, steps
     return {"value": 1.0 / t, "unit": "KDK steps/s", "cores": O.num_threads(), "kind": "port",
             "sample": f"median of {steps} timed steps after {warm} warm-up steps of the same {n}^3 FP64 workload "
                       f"with the oracle's C restatement (poisson + Timestep_criteria + Split_op_MPI, 6 3-D FFTs "
-                      f"per step), OpenMP", "step_s": [round(x, 3) for x in times[warm:]]}
+                      f"per step, each a plain radix-2 FFT per line with OpenMP over lines; not an FFTW-class "
+                      f"transform)", "step_s": [round(x, 3) for x in times[warm:]]}
+
+
+def small_mesh(F, steps: int = 200, cpu_steps: int = 10) -> dict:
+    """The 64^3 FP64 workload (Zel'dovich IC, box 10 Mpc/h, z = 50): the
+    library's adaptive loop on the GPU (one ifdm_simulate call of `steps`
+    steps, launch-latency bound at this size) and the oracle's C
+    restatement on the host cores (median of `cpu_steps` steps)."""
+    n = 64
+    psi = make_ic(F, n, 5.0)
+    with F.Solver(F.Config.from_cli(2.5e-24, 0.0, 5.0, n)) as s:
+        s.set_psi(psi)
+        a, _ = s.simulate_native(1.0 / 51, 5)
+        t0 = time.perf_counter()
+        s.simulate_native(a, steps)
+        gpu = steps / (time.perf_counter() - t0)
+    cpu = cpu_baseline(n, psi, 0.0, 5.0, cpu_steps)
+    return {"workload": "KDK step at 64^3, box 10 Mpc/h, z=50, mfdm=2.5e-24 eV, g=0, adaptive dt, FP64",
+            "gpu_steps_per_s": round(gpu, 2), "gpu_steps": steps, "cpu_baseline": cpu}
 
 
 def rel_l2(a, b) -> float:
@@ -327,7 +346,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
-    ap.add_argument("--prof-steps", type=int, default=5)
+    ap.add_argument("--prof-steps", type=int, default=10)
+    ap.add_argument("--no-small", action="store_true", help="skip the 64^3 GPU and CPU timing")
     ap.add_argument("--error-steps", type=str, default="100,1000",
                     help="comma list of cumulative step counts at which the mixed path's error against the "
                          "FP64 run is reported ('' to skip)")
@@ -449,11 +469,28 @@ def main():
     cps = [int(x) for x in args.error_steps.split(",") if x.strip()]
     if not args.no_mixed and cps:
         out["mixed"]["error_vs_fp64"] = error_study(F, cfg, cfgm, psi, sorted(cps), mode=args.error_schedule)
+    if not args.no_mixed:
+        # the mixed path's density error in one place: on the FP64 run's own
+        # adaptive steps (unsplit: the reference's step sizes) and after the
+        # 1000-step split schedule
+        m = out["mixed"]
+        head = {}
+        if "error_vs_fp64_after_steps" in m:
+            head["unsplit_steps"] = m["error_vs_fp64_after_steps"]["steps"]
+            head["density_rel_l2_unsplit"] = m["error_vs_fp64_after_steps"]["density_rel_l2"]
+        for row in m.get("error_vs_fp64", {}).get("checkpoints", []):
+            head[f"density_rel_l2_after_{row['steps']}_split_steps"] = row["mixed"]["density_rel_l2"]
+        if head:
+            m["density_error"] = head
     if args.adjoint_steps:
         out["adjoint"] = adjoint_study(F, n, xmax_in, g_in, args.adjoint_steps, args.adjoint_k, args.adjoint_fd)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(n, psi, g_in, xmax_in, args.cpu_steps)
+        if not args.no_small:
+            del psi
+            log("[bench] 64^3: GPU adaptive loop and CPU oracle ...")
+            out["mesh_64"] = small_mesh(F)
     print(json.dumps(out), flush=True)
 
 

@@ -1160,7 +1160,10 @@ int ifdm_set_profiling(ifdm_ctx *c, int on) {
     HIPCHK(c, hipSetDevice(c->cfg.device));
     if (on && c->ev_pool.empty()) {
         c->ev_pool.resize(IFDM_EV_POOL, nullptr);
-        for (auto &e : c->ev_pool) HIPCHK(c, hipEventCreate(&e));
+        // timing-only events: no system-scope fence (cache write-back and
+        // invalidation) when they complete, which would slow the launch after
+        // each of them and lengthen the times they bracket
+        for (auto &e : c->ev_pool) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     }
     c->prof = on != 0;
     c->ev_pass.clear();
@@ -1183,7 +1186,7 @@ int ifdm_time_passes(ifdm_ctx *c, const ifdm_dt *dt, int iters, float *ms_out) {
     HIPCHK(c, hipSetDevice(c->cfg.device));
     const double aphi = 0.5 * dt->dPE * c->m_over_hbar, arho = 0.5 * dt->dQI * c->g_over_hbar;
     hipEvent_t ev[IFDM_NPASS + 1];
-    for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+    for (auto &e : ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     std::vector<double> acc(IFDM_NPASS, 0.0);
     int rc = 0;
     for (int it = 0; it < iters && !rc; ++it) {
