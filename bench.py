@@ -66,12 +66,16 @@ def make_ic(F, n: int, xmax_in: float, seed: int = 1234) -> np.ndarray:
         return s.get_psi()
 
 
-def cpu_baseline(n: int, psi: np.ndarray, cfg_g_in: float, xmax_in: float, steps: int = 5, warm: int = 2) -> dict:
-    """The oracle's C restatement of the reference step on the host's cores:
-    `warm` untimed steps, then the median of `steps` timed ones."""
+def cpu_baseline(n: int, psi: np.ndarray, cfg_g_in: float, xmax_in: float, steps: int = 5, warm: int = 2,
+                 library_fft: bool = False) -> dict:
+    """The reference step on the host's cores, restated by the oracle:
+    `warm` untimed steps, then the median of `steps` timed ones.  Default:
+    the C restatement (plain radix-2 FFT per line, OpenMP); library_fft:
+    the same step over scipy.fft (pocketfft) with one worker per host
+    thread (oracle.FastState)."""
     from oracle import oracle as O
     rp = O.RunParams(n=n, mfdm=2.5e-24, g_in=cfg_g_in, xmax_in=xmax_in)
-    st = O.State(rp, psi)
+    st = O.FastState(rp, psi) if library_fft else O.State(rp, psi)
     a = 1.0 / 51
     times = []
     for i in range(warm + steps):
@@ -83,26 +87,35 @@ def cpu_baseline(n: int, psi: np.ndarray, cfg_g_in: float, xmax_in: float, steps
         times.append(time.perf_counter() - t0)
         log(f"[cpu baseline] step {i} {times[-1]:.2f} s")
     t = float(np.median(times[warm:]))
-    return {"value": 1.0 / t, "unit": "KDK steps/s", "cores": O.num_threads(), "kind": "port",
+    if library_fft:
+        cores = st.workers
+        st.close()
+        how = ("the oracle's restatement over scipy.fft (pocketfft, SIMD mixed-radix transforms, one worker per "
+               "host thread; elementwise work split over x-planes in a thread pool)")
+    else:
+        cores = O.num_threads()
+        how = ("the oracle's C restatement, each 3-D FFT a plain radix-2 FFT per line with OpenMP over lines "
+               "(not an FFTW-class transform)")
+    return {"value": 1.0 / t, "unit": "KDK steps/s", "cores": cores, "kind": "port",
             "sample": f"median of {steps} timed steps after {warm} warm-up steps of the same {n}^3 FP64 workload "
-                      f"with the oracle's C restatement (poisson + Timestep_criteria + Split_op_MPI, 6 3-D FFTs "
-                      f"per step, each a plain radix-2 FFT per line with OpenMP over lines; not an FFTW-class "
-                      f"transform)", "step_s": [round(x, 3) for x in times[warm:]]}
+                      f"(poisson + Timestep_criteria + Split_op_MPI, 6 3-D FFTs per step) with {how}",
+            "step_s": [round(x, 3) for x in times[warm:]]}
 
 
 def small_mesh(F, steps: int = 200, cpu_steps: int = 10) -> dict:
     """The 64^3 FP64 workload (Zel'dovich IC, box 10 Mpc/h, z = 50): the
-    library's adaptive loop on the GPU (one ifdm_simulate call of `steps`
-    steps, launch-latency bound at this size) and the oracle's C
+    library's adaptive loop on the GPU (`steps` steps resuming a run after 5
+    warm-up steps, launch-latency bound at this size) and the oracle's C
     restatement on the host cores (median of `cpu_steps` steps)."""
     n = 64
     psi = make_ic(F, n, 5.0)
     with F.Solver(F.Config.from_cli(2.5e-24, 0.0, 5.0, n)) as s:
         s.set_psi(psi)
-        a, _ = s.simulate_native(1.0 / 51, 5)
+        a, _ = s.simulate_native(1.0 / 51, 5, keep_pending=True)
         t0 = time.perf_counter()
-        s.simulate_native(a, steps)
+        s.simulate_native(a, steps, resume=True, keep_pending=True)
         gpu = steps / (time.perf_counter() - t0)
+        s.flush()
     cpu = cpu_baseline(n, psi, 0.0, 5.0, cpu_steps)
     return {"workload": "KDK step at 64^3, box 10 Mpc/h, z=50, mfdm=2.5e-24 eV, g=0, adaptive dt, FP64",
             "gpu_steps_per_s": round(gpu, 2), "gpu_steps": steps, "cpu_baseline": cpu}
@@ -113,27 +126,49 @@ def rel_l2(a, b) -> float:
 
 
 def timed_adaptive(F, cfg, psi, warmup: int, steps: int, prof_steps: int, keep_state: bool, persistent: int = -1):
-    """warm-up, then `steps` timed adaptive steps (ifdm_simulate, no events),
-    then `prof_steps` more with live per-launch HIP events for the per-pass
-    times.  Returns (steps/s, seconds, avg ms per pass, schedule of the
-    warm-up + timed steps, psi and phi after them (or None), a)."""
+    """`warmup` untimed adaptive steps (ifdm_simulate_ex), the last of them (up
+    to 3) with HIP events around every pass to find the dominant pass; then
+    `steps` timed steps carrying events around that pass only (its live
+    duration over the timed region); then `prof_steps` more with events
+    around every pass for the per-pass table.  Returns (steps/s, seconds, avg
+    ms per pass from the last phase, schedule of the warm-up + timed steps,
+    psi and phi after them (or None), a, dominant pass index, its avg ms over
+    the timed steps)."""
     s = F.Solver(cfg)
     if persistent >= 0:
         s.set_option("persistent", persistent)
     s.set_psi(psi)
-    a, sched_w = s.simulate_native(1.0 / 51, warmup)
+    # one run in pieces (ifdm_simulate_ex): the entry Poisson solve belongs
+    # to the first warm-up piece, each later piece resumes the run, so the
+    # timed piece holds exactly `steps` steps of the run's steady state
+    nsel = min(3, warmup)
+    started = warmup - nsel > 0
+    a, sched_w = s.simulate_native(1.0 / 51, warmup - nsel, keep_pending=True) if started else (1.0 / 51, [])
+    dom = 1                                   # pass A unless the warm-up profile says otherwise
+    if nsel:
+        s.set_profiling(True)
+        a, d = s.simulate_native(a, nsel, resume=started, keep_pending=True)
+        started = True
+        sched_w += d
+        ms_sum, cnt = s.get_profile()
+        dom = int(np.argmax(ms_sum[:-1]))
+    s.set_option("profile_mask", 1 << dom)
+    s.set_profiling(True)
     t0 = time.perf_counter()
-    a, sched_t = s.simulate_native(a, steps)
+    a, sched_t = s.simulate_native(a, steps, resume=started, keep_pending=True)
     el = time.perf_counter() - t0
-    psi_out = s.get_psi() if keep_state else None
+    ms_sum, cnt = s.get_profile()
+    dom_ms = ms_sum[dom] / max(cnt, 1)
+    psi_out = s.get_psi() if keep_state else None      # (applies the pending half-kick)
     phi_out = s.get_phi() if keep_state else None
+    s.set_option("profile_mask", (1 << F.NPASS) - 1)
     s.set_profiling(True)
     s.simulate_native(a, prof_steps)
     ms_sum, cnt = s.get_profile()
     s.set_profiling(False)
     s.close()
     avg = [m / max(cnt, 1) for m in ms_sum]
-    return steps / el, el, avg, sched_w + sched_t, psi_out, phi_out, a
+    return steps / el, el, avg, sched_w + sched_t, psi_out, phi_out, a, dom, dom_ms
 
 
 def _phase_errors(psi, psi64, w):
@@ -327,14 +362,17 @@ def counter_traffic(n: int, precision: str) -> dict:
     return out
 
 
-def roofline(n: int, avg: list, sc: int, sr: int, precision: str) -> dict:
+def roofline(n: int, dom: int, dom_ms: float, avg: list, sc: int, sr: int, precision: str) -> dict:
+    """The dominant pass: its algorithmic bytes over its average duration
+    from HIP events around it during the timed steps (dom_ms); beside it the
+    average from the steps profiled pass by pass afterwards."""
     pb = pass_bytes(n, sc, sr)
-    dom = int(np.argmax(avg[:-1]))         # the passes; the last entry is the reduction
-    achieved = pb[dom] / (avg[dom] * 1e-3) / 1e9
+    achieved = pb[dom] / (dom_ms * 1e-3) / 1e9
     traffic = counter_traffic(n, precision).get(PASS_NAMES[dom])
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": PASS_NAMES[dom],
-            "kernel_avg_ms": round(avg[dom], 4), "kernel_alg_bytes": pb[dom]}
+            "kernel_avg_ms": round(dom_ms, 4), "kernel_avg_ms_every_pass_profiled": round(avg[dom], 4),
+            "kernel_alg_bytes": pb[dom]}
 
 
 def main():
@@ -394,10 +432,10 @@ def main():
     cfg = F.Config.from_cli(2.5e-24, g_in, xmax_in, n, plane_pad=args.plane_pad)
     log(f"[bench] g = {g_in:.4e} eV cm^3")
 
-    sps, el, avg, sched, psi64, phi64, a = timed_adaptive(F, cfg, psi, args.warmup, args.steps,
-                                                          args.prof_steps,
-                                                          not (args.no_mixed or args.no_mixed_check),
-                                                          args.persistent)
+    sps, el, avg, sched, psi64, phi64, a, dom, dom_ms = timed_adaptive(F, cfg, psi, args.warmup, args.steps,
+                                                                       args.prof_steps,
+                                                                       not (args.no_mixed or args.no_mixed_check),
+                                                                       args.persistent)
     log(f"[bench] FP64: {args.steps} steps in {el:.3f} s ({sps:.2f} steps/s), a = {a:.5f}")
     pb = pass_bytes(n)
     b_alg = sum(pb)
@@ -411,7 +449,7 @@ def main():
         "config": {"workload": f"KDK step at {n}^3, box {2 * xmax_in:g} Mpc/h, z=50, mfdm=2.5e-24 eV, "
                                f"g={g_in:.3e} eV cm^3 (1% of gravity), adaptive dt, FP64",
                    "n": n, "precision": "fp64"},
-        "roofline": roofline(n, avg, 16, 8, "fp64"),
+        "roofline": roofline(n, dom, dom_ms, avg, 16, 8, "fp64"),
         # frac_of_8TBs: the step on the reference five-pass-drift schedule's
         # bytes, (11 + 8h) s_c N^3 per step at 8 TB/s; bytes_moved_*: this
         # schedule's own algorithmic bytes (fewer: three drift passes)
@@ -426,14 +464,14 @@ def main():
 
     if not args.no_mixed:
         cfgm = F.Config.from_cli(2.5e-24, g_in, xmax_in, n, precision=F.MIXED, plane_pad=args.plane_pad)
-        spsm, elm, avgm, _schm, _p, _f, am = timed_adaptive(F, cfgm, psi, args.warmup, args.steps,
-                                                            args.prof_steps, False, args.persistent)
+        spsm, elm, avgm, _schm, _p, _f, am, domm, domm_ms = timed_adaptive(F, cfgm, psi, args.warmup, args.steps,
+                                                                          args.prof_steps, False, args.persistent)
         log(f"[bench] mixed: {args.steps} steps in {elm:.3f} s ({spsm:.2f} steps/s)")
         pbm = pass_bytes(n, 8, 4)
         out["mixed"] = {
             "value": round(spsm, 3), "unit": "KDK steps/s", "ms_per_step": round(1e3 * elm / args.steps, 4),
             "speedup_vs_fp64": round(spsm / sps, 3), "dtype": "c64 storage/FP32 FFT, FP64 phases+sums",
-            "roofline": roofline(n, avgm, 8, 4, "mixed"),
+            "roofline": roofline(n, domm, domm_ms, avgm, 8, 4, "mixed"),
             "step_hbm_frac_of_8TBs": round(pencil_model_bytes(n, 8) * spsm / 1e9 / HBM_PEAK_GBS, 4),
             "bytes_moved_frac_of_8TBs": round(sum(pbm) * spsm / 1e9 / HBM_PEAK_GBS, 4),
             "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avgm)},
@@ -485,8 +523,10 @@ def main():
     if args.adjoint_steps:
         out["adjoint"] = adjoint_study(F, n, xmax_in, g_in, args.adjoint_steps, args.adjoint_k, args.adjoint_fd)
     if not args.no_cpu_baseline:
-        log("[bench] CPU baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(n, psi, g_in, xmax_in, args.cpu_steps)
+        log("[bench] CPU baseline (oracle over scipy.fft) ...")
+        out["cpu_baseline"] = cpu_baseline(n, psi, g_in, xmax_in, args.cpu_steps, library_fft=True)
+        log("[bench] CPU baseline (oracle C restatement, radix-2) ...")
+        out["cpu_baseline_radix2"] = cpu_baseline(n, psi, g_in, xmax_in, args.cpu_steps)
         if not args.no_small:
             del psi
             log("[bench] 64^3: GPU adaptive loop and CPU oracle ...")

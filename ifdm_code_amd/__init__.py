@@ -86,7 +86,7 @@ EXPORTS = [
     "ifdm_write_snapshot", "ifdm_read_snapshot",
     "ifdm_create", "ifdm_destroy", "ifdm_last_error", "ifdm_create_error", "ifdm_set_psi", "ifdm_get_psi",
     "ifdm_get_phi", "ifdm_set_phi", "ifdm_poisson", "ifdm_kick", "ifdm_kdk_step", "ifdm_step", "ifdm_flush", "ifdm_run", "ifdm_drift",
-    "ifdm_fft3d", "ifdm_output_trigger", "ifdm_simulate", "ifdm_time_passes",
+    "ifdm_fft3d", "ifdm_output_trigger", "ifdm_simulate", "ifdm_simulate_ex", "ifdm_time_passes",
     "ifdm_set_profiling", "ifdm_get_profile", "ifdm_stream_copy", "ifdm_set_option", "ifdm_timestep_criteria", "ifdm_scale_integral", "ifdm_constant", "ifdm_abi_version",
 ]
 
@@ -116,6 +116,7 @@ def lib():
     L.ifdm_flush.argtypes = [vp]
     L.ifdm_drift.argtypes = [vp, ctypes.c_double]
     L.ifdm_simulate.argtypes = [vp, ctypes.c_double, ip, dp, ctypes.POINTER(_Dt)]
+    L.ifdm_simulate_ex.argtypes = [vp, ctypes.c_double, ip, dp, ctypes.POINTER(_Dt), ip]
     L.ifdm_time_passes.argtypes = [vp, ctypes.POINTER(_Dt), ip, ctypes.POINTER(ctypes.c_float)]
     L.ifdm_fft3d.argtypes = [vp, ip]
     L.ifdm_power_spectrum.argtypes = [vp, ip, dp, dp, dp]
@@ -390,12 +391,14 @@ class Solver:
         self._chk(lib().ifdm_run(self._h, arr, m, sts), "ifdm_run")
         return [s.as_dict() for s in sts[:m]] if stats else None
 
-    def simulate_native(self, a0: float, nsteps: int):
-        """ifdm_simulate: the adaptive loop in the library with merged
-        half-kicks.  Returns (a_final, [(dKE, dPE, dQI), ...])."""
+    def simulate_native(self, a0: float, nsteps: int, resume: bool = False, keep_pending: bool = False):
+        """ifdm_simulate (ifdm_simulate_ex with resume / keep_pending): the
+        adaptive loop in the library with merged half-kicks.  Returns
+        (a_final, [(dKE, dPE, dQI), ...])."""
         sched = (_Dt * max(nsteps, 1))()
         a = ctypes.c_double()
-        self._chk(lib().ifdm_simulate(self._h, a0, nsteps, ctypes.byref(a), sched), "ifdm_simulate")
+        flags = (1 if resume else 0) | (2 if keep_pending else 0)
+        self._chk(lib().ifdm_simulate_ex(self._h, a0, nsteps, ctypes.byref(a), sched, flags), "ifdm_simulate_ex")
         return a.value, [(d.dKE, d.dPE, d.dQI) for d in sched[:nsteps]]
 
     def stream_copy(self, nbytes: int, iters: int = 10) -> float:

@@ -49,6 +49,9 @@ struct ifdm_ctx {
     // second half-kick of the last ifdm_step, not yet applied
     bool pending = false;
     double pend_aphi = 0.0, pend_arho = 0.0;
+    // reductions after the last ifdm_step (valid while its half-kick is pending)
+    ifdm_stats run_stats{};
+    bool run_stats_valid = false;
     // initial-condition grids (ifdm_ic_begin .. ifdm_ic_finish): the FP64
     // density and the velocity grids, float (ic_f32, the reference-order
     // deposit) or FP64 (the atomic deposit)
@@ -72,6 +75,7 @@ struct ifdm_ctx {
     int num_cu = 0;
     // optional live per-pass timing (ifdm_set_profiling)
     bool prof = false;
+    unsigned prof_mask = ~0u;                 // passes (bit = pass id) that live profiling brackets
     std::vector<hipEvent_t> ev_pool;          // 2 events per profiled launch
     std::vector<int> ev_pass;                 // pass id of each recorded launch
     int cur_pass = 0;                         // pass id of the launches being issued
@@ -115,7 +119,7 @@ static constexpr int IFDM_EV_POOL = 4096;
 // With live profiling on, brackets the next launch with a pair of events
 // from the pool and remembers its pass id.
 static inline int prof_begin(ifdm_ctx *c) {
-    if (!c->prof) return 0;
+    if (!c->prof || !((c->prof_mask >> c->cur_pass) & 1u)) return 0;
     const size_t k = c->ev_pass.size();
     if (2 * k + 1 >= c->ev_pool.size()) { c->err = "profiling event pool exhausted"; return IFDM_E_STATE; }
     HIPCHK(c, hipEventRecord(c->ev_pool[2 * k], c->stream));
@@ -123,7 +127,7 @@ static inline int prof_begin(ifdm_ctx *c) {
     return 0;
 }
 static inline int prof_end(ifdm_ctx *c) {
-    if (!c->prof) return 0;
+    if (!c->prof || !((c->prof_mask >> c->cur_pass) & 1u)) return 0;
     HIPCHK(c, hipEventRecord(c->ev_pool[2 * (c->ev_pass.size() - 1) + 1], c->stream));
     return 0;
 }

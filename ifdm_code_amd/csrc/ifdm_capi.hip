@@ -751,11 +751,16 @@ int ifdm_step(ifdm_ctx *c, const ifdm_dt *dt, ifdm_stats *st) {
         krho += c->pend_arho;
     }
     int rc;
+    c->run_stats_valid = false;
     if ((rc = d.kdp(c, kphi, krho, dt->dKE))) return rc;
     c->pending = true;
     c->pend_aphi = aphi;
     c->pend_arho = arho;
-    if (st && (rc = read_stats(c, st))) return rc;
+    if (st) {
+        if ((rc = read_stats(c, st))) return rc;
+        c->run_stats = *st;
+        c->run_stats_valid = true;
+    }
     return accumulate_profile(c);
 }
 
@@ -784,11 +789,21 @@ int ifdm_run(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, ifdm_stats *st) {
     return ifdm_flush(c);
 }
 
-int ifdm_simulate(ifdm_ctx *c, double a0, int nsteps, double *a_out, ifdm_dt *sched_out) {
-    if (!c || nsteps < 0 || !(a0 > 0.0)) return IFDM_E_ARG;
+int ifdm_simulate_ex(ifdm_ctx *c, double a0, int nsteps, double *a_out, ifdm_dt *sched_out, int flags) {
+    if (!c || nsteps < 0 || !(a0 > 0.0) || (flags & ~(IFDM_SIM_RESUME | IFDM_SIM_KEEP_PENDING))) return IFDM_E_ARG;
     int rc;
     ifdm_stats st;
-    if ((rc = ifdm_poisson(c, &st))) return rc;                  // :218 of the first step
+    if (flags & IFDM_SIM_RESUME) {
+        // inside a run: phi and the reductions are those of the last step,
+        // whose second half-kick is still pending (it leaves |psi| alone)
+        if (!c->pending || !c->run_stats_valid) {
+            c->err = "ifdm_simulate_ex: IFDM_SIM_RESUME needs the pending half-kick of a previous step";
+            return IFDM_E_STATE;
+        }
+        st = c->run_stats;
+    } else if ((rc = ifdm_poisson(c, &st))) {
+        return rc;                                               // :218 of the first step
+    }
     double a = a0;
     for (int s = 0; s < nsteps; ++s) {
         ifdm_dt dt;
@@ -801,9 +816,13 @@ int ifdm_simulate(ifdm_ctx *c, double a0, int nsteps, double *a_out, ifdm_dt *sc
         a = an;
         if ((rc = ifdm_step(c, &dt, &st))) return rc;
     }
-    if ((rc = ifdm_flush(c))) return rc;
+    if (!(flags & IFDM_SIM_KEEP_PENDING) && (rc = ifdm_flush(c))) return rc;
     if (a_out) *a_out = a;
     return IFDM_OK;
+}
+
+int ifdm_simulate(ifdm_ctx *c, double a0, int nsteps, double *a_out, ifdm_dt *sched_out) {
+    return ifdm_simulate_ex(c, a0, nsteps, a_out, sched_out, 0);
 }
 
 int ifdm_internal_theta_solve(ifdm_ctx *c, void *buf64, double coef) {
@@ -1143,6 +1162,14 @@ int ifdm_set_option(ifdm_ctx *c, const char *name, long value) {
     if (std::strcmp(name, "persistent") == 0) {
         if (value < 0 || value > 31) { c->err = "persistent must be a mask in 0..31"; return IFDM_E_ARG; }
         c->persistent = (int)value;
+        return IFDM_OK;
+    }
+    if (std::strcmp(name, "profile_mask") == 0) {
+        if (value <= 0 || value >= (1L << IFDM_NPASS)) {
+            c->err = "profile_mask must be a non-empty mask of the " + std::to_string(IFDM_NPASS) + " passes";
+            return IFDM_E_ARG;
+        }
+        c->prof_mask = (unsigned)value;
         return IFDM_OK;
     }
     if (std::strcmp(name, "ic_deposit") == 0) {

@@ -134,6 +134,18 @@ int ifdm_run(ifdm_ctx *ctx, const ifdm_dt *sched, int nsteps, ifdm_stats *stats_
  * (nsteps entries, may be NULL): the integrals used, for replay. */
 int ifdm_simulate(ifdm_ctx *ctx, double a0, int nsteps, double *a_out, ifdm_dt *sched_out);
 
+/* ifdm_simulate with flags, so one run can be driven in pieces (to time a
+ * stretch of it, or to return to the caller between steps without a
+ * flush): IFDM_SIM_KEEP_PENDING leaves the last step's second half-kick
+ * pending (no ifdm_flush at the end); IFDM_SIM_RESUME continues such a run
+ * without the entry Poisson solve, taking phi and the reductions from its
+ * last step (IFDM_E_STATE when no half-kick is pending).  Calls with
+ * KEEP_PENDING then RESUME give the same states as one call over all the
+ * steps.  flags = 0 is ifdm_simulate. */
+#define IFDM_SIM_RESUME 1
+#define IFDM_SIM_KEEP_PENDING 2
+int ifdm_simulate_ex(ifdm_ctx *ctx, double a0, int nsteps, double *a_out, ifdm_dt *sched_out, int flags);
+
 /* Times each pass of one step with HIP events on the context's stream,
  * averaged over iters repetitions (ms_out[IFDM_NPASS]: the drift table; the
  * passes A, B, C - the y, x, z drift round trips, A with the kick, C with
@@ -163,7 +175,10 @@ int ifdm_set_option(ifdm_ctx *ctx, const char *name, long value);
 /* Live per-pass timing: with on != 0 every step run by ifdm_run or
  * ifdm_simulate brackets each pass with HIP events on the context's stream
  * and accumulates the durations (reset by each call of this function).
- * ifdm_get_profile returns the summed ms per pass and the number of steps. */
+ * ifdm_get_profile returns the summed ms per pass and the number of steps.
+ * Option "profile_mask" (ifdm_set_option; bit p = pass p of ifdm_time_passes'
+ * order, default all) limits the brackets to the chosen passes, so a timed
+ * run carries two events per step around one pass only. */
 int ifdm_set_profiling(ifdm_ctx *ctx, int on);
 int ifdm_get_profile(const ifdm_ctx *ctx, double *ms_sum, long *count);
 

@@ -210,6 +210,102 @@ class State:
         return {"max_phi": out[0], "max_rho": out[1], "norm": out[2]}
 
 
+class FastState:
+    """The same FP64 step (poisson, kick, drift, Split_op_MPI, the two scans
+    of Timestep_criteria) over an FFT library: scipy.fft (pocketfft, SIMD
+    radix kernels) with `workers` threads for the 3-D transforms and the
+    elementwise work split over x-planes in a thread pool.  A CPU baseline
+    of the reference's algorithm on a tuned transform, beside the C
+    restatement's plain radix-2 one; pinned against State (the C oracle) by
+    tests/test_oracle_physics.py.  phi is held unpadded (N^3 doubles)."""
+
+    def __init__(self, rp: RunParams, psi: np.ndarray, workers: int = 0):
+        from concurrent.futures import ThreadPoolExecutor
+        n = rp.n
+        self.rp = rp
+        self.psi = np.ascontiguousarray(psi, dtype=np.complex128).reshape(n, n, n).copy()
+        self.phi = np.zeros((n, n, n))
+        self.workers = workers or (os.cpu_count() or 1)
+        self._pool = ThreadPoolExecutor(self.workers)
+        self._kax = kvec(n, rp.xmax) ** 2                                  # pow(k, 2), struct3D.h:76-85
+        self._mt = rp.mfdm / math.pow(c_light, 2)
+
+    def close(self):
+        self._pool.shutdown()
+
+    def _planes(self, fn):
+        n = self.rp.n
+        step = max(1, -(-n // (4 * self.workers)))
+        list(self._pool.map(lambda i0: fn(i0, min(n, i0 + step)), range(0, n, step)))
+
+    def _ksq(self, i0, i1, nz):
+        k = self._kax
+        return k[i0:i1, None, None] + k[None, :, None] + k[None, None, :nz]
+
+    def poisson(self):
+        import scipy.fft as sf
+        rp, n = self.rp, self.rp.n
+        ms = rp.mfdm_solar
+
+        def src(i0, i1):
+            z = self.psi[i0:i1]
+            self.phi[i0:i1] = ms * (z.real * z.real + z.imag * z.imag) - rp.rhoc      # :88
+        self._planes(src)
+        sk = sf.rfftn(self.phi, workers=self.workers)                                  # :94
+        f = -4.0 * math.pi * bigG                                                       # :105
+        h = n // 2 + 1
+
+        def green(i0, i1):
+            kk = self._ksq(i0, i1, h)
+            with np.errstate(divide="ignore"):
+                g = np.where(kk == 0, 0.0, f / kk)                                     # :101-108
+            sk[i0:i1] *= g
+        self._planes(green)
+        self.phi = sf.irfftn(sk, s=(n, n, n), workers=self.workers)                    # :111-116 (norm 1/N^3)
+
+    def kick(self, dPE: float, dQI: float):
+        rp = self.rp
+        cphi = dPE * self._mt / hbar
+        crho = dQI * rp.g / hbar
+
+        def k(i0, i1):
+            z = self.psi[i0:i1]
+            th = -0.5 * (cphi * self.phi[i0:i1] + crho * (z.real * z.real + z.imag * z.imag))   # :130-134
+            self.psi[i0:i1] = z * (np.cos(th) + 1j * np.sin(th))
+        self._planes(k)
+
+    def drift(self, dKE: float):
+        import scipy.fft as sf
+        n = self.rp.n
+        ke = 0.5 * hbar / self._mt                                                     # struct3D.h:121
+        a = sf.fftn(self.psi, workers=self.workers, overwrite_x=True)                  # :136
+
+        def d(i0, i1):
+            th = -ke * dKE * self._ksq(i0, i1, n)
+            a[i0:i1] *= np.cos(th) + 1j * np.sin(th)                                   # :141 (N^-3 in ifftn)
+        self._planes(d)
+        self.psi = sf.ifftn(a, workers=self.workers, overwrite_x=True)                 # :144
+
+    def split_op(self, dKE: float, dPE: float, dQI: float):
+        self.kick(dPE, dQI)
+        self.drift(dKE)
+        self.poisson()
+        self.kick(dPE, dQI)
+
+    def maxima(self):
+        """max|phi| over every cell, max|psi|^2 and sum|psi|^2 (:26-27)."""
+        n = self.rp.n
+        part = {}
+
+        def m(i0, i1):
+            z = self.psi[i0:i1]
+            rr = z.real * z.real + z.imag * z.imag
+            part[i0] = (float(np.max(np.abs(self.phi[i0:i1]))), float(np.max(rr)), float(np.sum(rr)))
+        self._planes(m)
+        v = [part[k] for k in sorted(part)]
+        return {"max_phi": max(x[0] for x in v), "max_rho": max(x[1] for x in v), "norm": sum(x[2] for x in v)}
+
+
 # ----------------------------------------------------------- time step
 def _Ha(x, n):
     """Ha{2,3,4}_preintegrat, src/struct3D.h:127-146"""

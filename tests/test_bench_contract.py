@@ -49,11 +49,13 @@ def test_counter_traffic_covers_every_pass(precision):
 
 
 def test_roofline_object():
-    """roofline() names the slowest pass and prices its algorithmic bytes
-    against 8 TB/s."""
-    avg = [0.006, 1.1, 0.9, 1.2, 0.4, 0.46, 0.4, 0.38, 0.01]
-    r = bench.roofline(512, avg, 16, 8, "fp64")
+    """roofline() prices the dominant pass's algorithmic bytes over its live
+    duration in the timed steps against 8 TB/s, and reports the average of
+    the pass-by-pass profile beside it."""
+    avg = [0.006, 1.1, 0.9, 1.25, 0.4, 0.46, 0.4, 0.38, 0.01]
+    r = bench.roofline(512, 3, 1.2, avg, 16, 8, "fp64")
     assert r["kernel"] == "C_zdrift_src_r2c" and r["bound"] == "hbm" and r["peak"] == 8000.0
+    assert r["kernel_avg_ms"] == 1.2 and r["kernel_avg_ms_every_pass_profiled"] == 1.25
     alg = bench.pass_bytes(512)[3]
     assert r["achieved"] == pytest.approx(alg / 1.2e-3 / 1e9, rel=1e-3)
     assert r["frac"] == pytest.approx(r["achieved"] / 8000.0, rel=1e-3)

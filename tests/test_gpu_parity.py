@@ -492,3 +492,32 @@ def test_poisson_cosine_large(n, precision):
     # the lattice phases are 2 pi (3i - 5j + 7k + 2.5) / n: the nearest to a
     # multiple of 2 pi is pi / n away
     assert st["max_phi"] == pytest.approx(abs(C) * math.cos(math.pi / n), rel=1e-9 if precision == 0 else 1e-5)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_simulate_in_pieces_equals_one_call(prec):
+    """ifdm_simulate_ex: a run driven in pieces (IFDM_SIM_KEEP_PENDING, then
+    IFDM_SIM_RESUME, which takes phi and the reductions of the last step
+    instead of a new entry solve) gives bit for bit the schedule and state of
+    one ifdm_simulate call over all the steps; RESUME without a pending
+    half-kick is refused."""
+    F, _O = _mods()
+    n = 32
+    cfg = F.Config.from_cli(2.5e-24, G_IN, 5.0, n, precision=prec)
+    psi = make_psi(n, 12, cfg.mfdm_solar)
+    with F.Solver(cfg) as s:
+        s.set_psi(psi)
+        a1, sched1 = s.simulate_native(1.0 / 51, 9)
+        one = s.get_psi()
+    with F.Solver(cfg) as s:
+        s.set_psi(psi)
+        with pytest.raises(F.IfdmError, match="IFDM_SIM_RESUME"):
+            s.simulate_native(1.0 / 51, 2, resume=True)
+        a, sched = s.simulate_native(1.0 / 51, 4, keep_pending=True)
+        a, d = s.simulate_native(a, 3, resume=True, keep_pending=True)
+        sched += d
+        a, d = s.simulate_native(a, 2, resume=True)
+        sched += d
+        pieces = s.get_psi()
+    assert a == a1 and sched == sched1
+    assert np.array_equal(pieces, one)

@@ -160,3 +160,41 @@ def test_integrals_vs_mpmath(n_pow, a0, a1):
     f = lambda x: 1 / (H0 * mp.sqrt(Om / x ** 3 + (1 - Om)) * x ** n_pow)
     ref = float(mp.quad(f, [a0, a1]))
     assert O.integral(a0, a1, n_pow) == pytest.approx(ref, rel=1e-13)
+
+
+def test_fft_library_restatement_closed_forms_and_oracle():
+    """FastState (the step over scipy.fft, the bench's CPU baseline on a
+    library transform) against the same closed forms as the C oracle - the
+    single-mode potential and the plane-wave drift - and against the C
+    oracle (State) over three adaptive steps with g != 0."""
+    n = 32
+    rp = O.RunParams(n=n, g_in=4e-55)
+    i, j, k = _grid_index(n)
+    m = (2, 1, 3)
+    arg = 2 * np.pi * (m[0] * i + m[1] * j + m[2] * k) / n
+    rho = rp.rhoc * (1 + 0.03 * np.cos(arg))
+    fs = O.FastState(rp, np.sqrt(rho / rp.mfdm_solar), workers=4)
+    fs.poisson()
+    L = 2 * rp.xmax
+    k0sq = sum((2 * np.pi * mm / L) ** 2 for mm in m)
+    assert rel_l2(fs.phi, -4 * np.pi * O.bigG * rp.rhoc * 0.03 * np.cos(arg) / k0sq) < 1e-12
+    wave = np.exp(2j * np.pi * (3 * i - 2 * j + 5 * k) / n)
+    ksq = (2 * np.pi / L) ** 2 * 38
+    fs.psi = wave.copy()
+    fs.drift(2.0e4)
+    assert rel_l2(fs.psi, wave * np.exp(-1j * O.hbar / (2 * rp.mfdm / O.c_light ** 2) * ksq * 2.0e4)) < 1e-13
+    fs.close()
+    psi = make_psi(n, 6, rp.mfdm_solar)
+    a, b = O.State(rp, psi), O.FastState(rp, psi, workers=4)
+    t = 1.0 / 51
+    for _ in range(3):
+        a.poisson()
+        b.poisson()
+        ma, mb = a.maxima(), b.maxima()
+        for key in ("max_phi", "max_rho", "norm"):
+            assert mb[key] == pytest.approx(ma[key], rel=1e-12)
+        dKE, dPE, dQI, t, _w = O.timestep_criteria(rp, t, ma["max_phi"], ma["max_rho"])
+        a.split_op(dKE, dPE, dQI)
+        b.split_op(dKE, dPE, dQI)
+    assert rel_l2(b.psi, a.psi) < 1e-13
+    b.close()
