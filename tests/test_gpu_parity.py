@@ -521,3 +521,40 @@ def test_simulate_in_pieces_equals_one_call(prec):
         pieces = s.get_psi()
     assert a == a1 and sched == sched1
     assert np.array_equal(pieces, one)
+
+
+def test_mixed_set_psi_device_refreshes_mean_density():
+    """In mixed contexts the Poisson source subtracts the FP64 mean density
+    m_s <|psi|^2>.  ifdm_set_psi_device computes it from the new field, as
+    ifdm_set_psi does: loading a field of 4x the density from device memory
+    after another field gives the potential of the host path.  The k = 0
+    mode is zeroed either way: a stale mean costs FP32 precision in the
+    source, m_s|psi|^2 - rho_mean rounded to float, and with a 0.1 % density
+    contrast a mean off by 3 rho (the first field's) would leave rounding
+    noise near 5e-5 of the fluctuation, against 1e-7 here."""
+    import ctypes
+    F, _O = _mods()
+    n = 32
+    cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n, precision=F.MIXED)
+    psi1 = make_psi(n, 4, cfg.mfdm_solar, delta_rms=0.001)
+    psi2 = 2.0 * make_psi(n, 5, cfg.mfdm_solar, delta_rms=0.001)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    dev = ctypes.c_void_p()
+    nbytes = psi2.nbytes
+    assert hip.hipMalloc(ctypes.byref(dev), ctypes.c_size_t(nbytes)) == 0
+    try:
+        assert hip.hipMemcpy(dev, psi2.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(nbytes), 1) == 0
+        with F.Solver(cfg) as s:
+            s.set_psi(psi1)
+            s.poisson()
+            s.set_psi_device(dev.value)
+            st_dev = s.poisson()
+            phi_dev = s.get_phi()
+        with F.Solver(cfg) as s:
+            s.set_psi(psi2)
+            st_host = s.poisson()
+            phi_host = s.get_phi()
+    finally:
+        hip.hipFree(dev)
+    assert st_dev["norm"] == pytest.approx(st_host["norm"], rel=1e-12)
+    assert rel_l2(phi_dev, phi_host) < 1e-7
