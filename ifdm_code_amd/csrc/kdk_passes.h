@@ -119,6 +119,14 @@ template <typename T, int N, bool WIDE = false, int AXIS = 1, int MODE = -1> con
     return N == 1024 && !WIDE && AXIS == 1 && (sizeof(T) == 4 || MODE == S_KICK_DRIFT1 || MODE == S_DRIFT1);
 }
 
+// Schedule variant of a strided pass's transforms (Sched): radix-16 stages
+// where they exist (1024 points at 16 per thread), except in the kick pass.
+// Measured at 1024^3 mixed (ms, radix <= 8 -> radix 16): B 5.76 -> 5.30,
+// G 3.13 -> 2.84, F 1.86 -> 1.85, H 1.91 -> 1.88; the kick pass A, capped
+// at 128 VGPRs, 7.43 -> 7.72, so it keeps radices <= 8 (as do the z passes,
+// whose pass C took 205 VGPRs with radix 16: 4.84 -> 5.09).
+template <int MODE> constexpr int sp_sched() { return MODE == S_KICK_DRIFT1 ? 0 : 1; }
+
 // z-columns per strided-pass tile: 128-byte row segments (8 complex
 // doubles, 16 complex floats; 64-byte ones for narrow tiles); WIDE doubles
 // them for FP64 (256-byte rows, used by the 3-D transform's x pass), and a
@@ -129,7 +137,7 @@ template <typename T, int N, bool WIDE = false, int MODE = -1, int AXIS = 1> con
     constexpr int want = (WIDE && sizeof(cx<T>) == 16) ? 2 * base : base;
     constexpr int cap = 1024 / (N / SchedT<T, N>::E);
     // and the exchange buffer within the 160 KB of LDS
-    constexpr int lcap = 163840 / (lds_col_line<N>() * (int)sizeof(cx<T>));
+    constexpr int lcap = 163840 / (lds_col_line<N, narrow, pad_shift_t<T, N, sp_sched<MODE>()>()>() * (int)sizeof(cx<T>));
     int c = want < cap ? want : cap;
     while (c > lcap) c /= 2;
     return c;
@@ -220,7 +228,7 @@ template <typename T, int N, int MODE>
 __global__ __launch_bounds__((z_lines<T, N>() * (N / SchedT<T, N>::E)))
 void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ shat,
              const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, ZArgs za, double *__restrict__ part) {
-    constexpr int E = SchedT<T, N>::E, P = N / E, LZ = z_lines<T, N>(), LS = lds_line<N>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, LZ = z_lines<T, N>(), LS = lds_line<N, pad_shift_t<T, N>()>();
     constexpr int HS = hs_of(N), NT = LZ * P;
     __shared__ cx<T> lds[LZ * LS];
     const int tid = threadIdx.x, l = tid / P, t = tid - l * P;
@@ -356,7 +364,7 @@ template <typename T, int N>
 __global__ __launch_bounds__((z_lines<T, N>() * (N / SchedT<T, N>::E)))
 void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, ZArgs za,
                 double *__restrict__ part) {
-    constexpr int H = N / 2, E = SchedT<T, H>::E, P = H / E, LZ = c2r_lines<T, N>(), LS = lds_line<H>();
+    constexpr int H = N / 2, E = SchedT<T, H>::E, P = H / E, LZ = c2r_lines<T, N>(), LS = lds_line<H, pad_shift_t<T, H>()>();
     constexpr int HS = hs_of(N), NT = LZ * P;
     static_assert(NT == z_lines<T, N>() * (N / SchedT<T, N>::E), "block shape");
     __shared__ cx<T> lds[(LZ * LS > NT / 2 + 1) ? LZ * LS : NT / 2 + 1];
@@ -421,8 +429,8 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
 // the transform (and the spectral multiply of passes C and G) of one tile;
 // FRESH: the first transform of the work-group (no earlier LDS use).  tab
 // and k2 are the drift table and k_i^2 (global memory, or LDS copies).
-template <typename T, int N, int MODE, bool FRESH, int AXIS, typename W>
-__device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const TwBase<T, N, W> &tw,
+template <typename T, int N, int MODE, bool FRESH, int AXIS, typename W, int V>
+__device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const TwBase<T, N, W, V> &tw,
                                               const SArgs &sa, int other, int k, bool valid,
                                               const cx<T> *tab, const double *k2) {
     constexpr int E = SchedT<T, N>::E, P = N / E;
@@ -556,7 +564,8 @@ template <int MODE> constexpr int xcd_group() { return MODE == S_KICK_DRIFT1 ? 4
 template <typename T, int N, int AXIS, int MODE, bool WIDE>
 __global__ __launch_bounds__((s_cols<T, N, WIDE, MODE, AXIS>() * (N / SchedT<T, N>::E)), (sp_wpe<T, MODE, N, WIDE, AXIS>()))
 void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa) {
-    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE, MODE, AXIS>(), LS = lds_col_line<N>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE, MODE, AXIS>();
+    constexpr int SV = sp_sched<MODE>(), LS = lds_col_line<N, s_narrow<T, N, WIDE, AXIS, MODE>(), pad_shift_t<T, N, SV>()>();
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     int bx = blockIdx.x, by = blockIdx.y;
@@ -576,7 +585,7 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
     const TileIO<T, N, ROW128 && !(WIDE && sizeof(T) == 8), ROW128> io(a, base0);
     cx<T> *L = lds + c * LS;
     constexpr bool PSI = MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1;
-    TwBase<T, N, tw_base_t<T, N, PSI>> tb;
+    TwBase<T, N, tw_base_t<T, N, PSI>, SV> tb;
     if constexpr (acc_tw<T, N>() && PSI) load_twiddles<T, N>(tb, t, tw64);
     else load_twiddles<T, N>(tb, t, tw);
     cx<T> v[E];
@@ -607,7 +616,8 @@ template <typename T, int N, int AXIS, int MODE>
 __global__ __launch_bounds__((s_cols<T, N, false, MODE, AXIS>() * (N / SchedT<T, N>::E)))
 void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa, int ntz,
                 int ntiles) {
-    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, false, MODE, AXIS>(), LS = lds_col_line<N>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, false, MODE, AXIS>(), SV = sp_sched<MODE>();
+    constexpr int LS = lds_col_line<N, false, pad_shift_t<T, N, SV>()>();
     constexpr int NT = TC * P;
     __shared__ cx<T> lds[TC * LS];
     // the spectral multiply's table in LDS, so no global load is issued
@@ -639,7 +649,7 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__r
     // the wait for the prefetch then lets this tile's stores stay in flight.
     constexpr bool FULL = N >= TC;
     constexpr bool PSI = MODE == S_DRIFT1;
-    TwBase<T, N, tw_base_t<T, N, PSI>> tb;
+    TwBase<T, N, tw_base_t<T, N, PSI>, SV> tb;
     if constexpr (acc_tw<T, N>() && PSI) load_twiddles<T, N>(tb, t, tw64);
     else load_twiddles<T, N>(tb, t, tw);
     cx<T> v[E], nx[E];
