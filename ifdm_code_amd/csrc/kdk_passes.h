@@ -224,8 +224,16 @@ template <typename T> __device__ __forceinline__ cx<T> stream_load(const cx<T> *
 }
 
 // ------------------------------------------------------------------ z passes
+// Minimum waves per SIMD asked of the compiler for a z pass: 3 for the FP64
+// pass C at N = 512 (176 -> 168 VGPRs, 20 bytes of spill; measured 1.09 ->
+// 1.03 ms; 4 waves spill 116 bytes, 1.32 ms).  At N = 1024 the same cap
+// spills 52 bytes and is slower (10.1 -> 10.9 ms).
+template <typename T, int N, int MODE> constexpr int zp_wpe() {
+    return (sizeof(T) == 8 && N == 512 && MODE == Z_DRIFT_SRC) ? 3 : 1;
+}
+
 template <typename T, int N, int MODE>
-__global__ __launch_bounds__((z_lines<T, N>() * (N / SchedT<T, N>::E)))
+__global__ __launch_bounds__((z_lines<T, N>() * (N / SchedT<T, N>::E)), (zp_wpe<T, N, MODE>()))
 void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ shat,
              const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, ZArgs za, double *__restrict__ part) {
     constexpr int E = SchedT<T, N>::E, P = N / E, LZ = z_lines<T, N>(), LS = lds_line<N, pad_shift_t<T, N>()>();
@@ -429,7 +437,10 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
 // the transform (and the spectral multiply of passes C and G) of one tile;
 // FRESH: the first transform of the work-group (no earlier LDS use).  tab
 // and k2 are the drift table and k_i^2 (global memory, or LDS copies).
-template <typename T, int N, int MODE, bool FRESH, int AXIS, typename W, int V>
+// HALF: tab holds the drift table for i <= N/2 only, and point i takes entry
+// min(i, N - i) (k^2 of i and N - i are the same double: kvec(i) for i >= N/2
+// is the exact negation of kvec(N - i), so the table entries are equal).
+template <typename T, int N, int MODE, bool FRESH, int AXIS, typename W, int V, bool HALF = false>
 __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const TwBase<T, N, W, V> &tw,
                                               const SArgs &sa, int other, int k, bool valid,
                                               const cx<T> *tab, const double *k2) {
@@ -442,7 +453,10 @@ __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const T
         // spectral indices of this thread's points: along the line i, fixed `other` and k
         if constexpr (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = cmul(v[e], tab[t + e * P]);
+            for (int e = 0; e < E; ++e) {
+                const int i = t + e * P;
+                v[e] = cmul(v[e], tab[HALF ? (i <= N / 2 ? i : N - i) : i]);
+            }
         } else {
             if constexpr (sizeof(T) == 4) {
                 // complex floats: the Green factor in FP32 (an approximate
@@ -541,6 +555,19 @@ template <typename T, int MODE, int N, bool WIDE = false, int AXIS = 1> constexp
     return (MODE == S_GREEN && sizeof(T) == 4 && N >= 128 && N <= 512) ? 8
                                                                           : (s_narrow<T, N, WIDE, AXIS, MODE>() ? 4 : 1);
 }
+// The kick pass's phi loads.  Batched (0 < PB): each batch of PB loads is
+// issued together and unconditionally, then its kicks; the batches are kept
+// apart by a scheduling barrier.  A load under a condition is a branch region
+// of its own, and the compiler then waits for each before issuing the next
+// (eight or sixteen memory round trips in series per tile).  Measured: FP64
+// A 1.12 -> 1.00 ms at 512^3; complex floats at 1024^3 with batches of 8
+// (of 16) 7.45 -> 6.70 ms (batches of 16: 6.92).  Complex floats up to
+// N = 512 keep the conditional loads (PB = 0): their pass holds 8 waves per
+// SIMD, which hide the round trips, and loads issued together take 78 VGPRs
+// (one work-group per CU: 0.71 -> 0.84 ms; capped at 64, they spill: 0.80).
+template <typename T, int N> constexpr int kick_pb() {
+    return sizeof(T) == 8 ? SchedT<T, N>::E : (N == 1024 ? 8 : 0);
+}
 
 // Narrow tiles: the grid is (a multiple of G column tiles, lines of the
 // fixed axis).  Within each run of 8G linear ids, id 8Gg + 8h + x takes the
@@ -575,6 +602,19 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
     } else if constexpr (MODE == S_KICK_DRIFT1 && (N / TC) % 16 == 0) {
         xcd_pair(bx, by);
     }
+    // 1024-point complex-float drift passes: the drift table in LDS.  From
+    // global memory its 16 loads per thread, issued after the forward
+    // transform under the 128-VGPR cap, were each awaited before the next
+    // (16 L2 round trips per tile).  Narrow tiles keep entries 0..N/2 (see
+    // spass_compute: two narrow work-groups per CU leave no room for all N).
+    // The forward transform's exchanges order these writes before the reads.
+    constexpr bool LTAB = (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1) && N == 1024 && sizeof(T) == 4;
+    constexpr bool HALF = LTAB && s_narrow<T, N, WIDE, AXIS, MODE>();
+    constexpr int TABN = LTAB ? (HALF ? N / 2 + 1 : N) : 1;
+    __shared__ cx<T> tabL[TABN];
+    if constexpr (LTAB) {
+        for (int i = (int)threadIdx.x; i < TABN; i += TC * P) tabL[i] = ((const cx<T> *)sa.dtab)[i];
+    }
     const int k0 = bx * TC, k = k0 + c;
     const int other = sa.other0 + block_order(sa.rev, by, gridDim.y);
     const bool valid = k < kv;
@@ -592,15 +632,37 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = valid ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
     if constexpr (MODE == S_KICK_DRIFT1) {
+        // phi loads: see kick_pb.  Batched, a column past kv reads the
+        // tile's first column, which lies inside the row (k0 < kv), and its
+        // value is not used.
         const T *ph = (const T *)sa.phi + base0;
+        constexpr int PB = kick_pb<T, N>();
+        if constexpr (PB == 0) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const double f = valid ? (double)ph[c + (long)(t + e * P) * stride] : 0.0;
-            const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
-            v[e] = cmul(v[e], expmi<T>(sa.aphi * f + sa.arho * rr));
+            for (int e = 0; e < E; ++e) {
+                const double f = valid ? (double)ph[c + (long)(t + e * P) * stride] : 0.0;
+                const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
+                v[e] = cmul(v[e], expmi<T>(sa.aphi * f + sa.arho * rr));
+            }
+        } else {
+            const int cp = valid ? c : 0;
+#pragma unroll
+            for (int e0 = 0; e0 < E; e0 += PB) {
+                T f[PB];
+#pragma unroll
+                for (int i = 0; i < PB; ++i) f[i] = ph[cp + (long)(t + (e0 + i) * P) * stride];
+#pragma unroll
+                for (int i = 0; i < PB; ++i) {
+                    const int e = e0 + i;
+                    const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
+                    v[e] = cmul(v[e], expmi<T>(sa.aphi * (valid ? (double)f[i] : 0.0) + sa.arho * rr));
+                }
+                if (PB < E) __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
-    spass_compute<T, N, MODE, true, AXIS>(v, L, t, tb, sa, other, k, valid, (const cx<T> *)sa.dtab, sa.k2);
+    spass_compute<T, N, MODE, true, AXIS, tw_base_t<T, N, PSI>, SV, HALF>(v, L, t, tb, sa, other, k, valid,
+                                                                        LTAB ? (const cx<T> *)tabL : (const cx<T> *)sa.dtab, sa.k2);
     if (valid) {
 #pragma unroll
         for (int e = 0; e < E; ++e) io.store(c + (long)(t + e * P) * stride, v[e]);
