@@ -36,15 +36,25 @@ struct PhaseRed {
 };
 
 __host__ __device__ __forceinline__ PhaseRed reduce_pio2(double x) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
     const double two_over_pi = 0.6366197723675814;
     const double p1 = 1.5707963267948966, p2 = 6.123233995736766e-17, p3 = -1.4973849048591698e-33;
-    const double n = rint(x * two_over_pi);
+    // 1.5 * 2^52: adding it rounds to the nearest integer (ties to even, as
+    // rint) and leaves that integer, two's complement, in the low bits of the
+    // significand for |x 2/pi| < 2^51; subtracting it again is exact.  The
+    // quadrant is then read from the bits, one add instead of a conversion
+    // of the double to a 64-bit integer.
+    const double magic = 6755399441055744.0;
+    const double t = x * two_over_pi + magic;
+    const double n = t - magic;
     double r = fma(-n, p1, x);
     r = fma(-n, p2, r);
     r = fma(-n, p3, r);
     PhaseRed o;
     o.r = r;
-    o.q = ((int)(long long)n) & 3;
+    o.q = (int)__builtin_bit_cast(long long, t) & 3;
     return o;
 }
 
@@ -94,13 +104,14 @@ __host__ __device__ __forceinline__ void expmi_parts(double theta, T *re, T *im)
     const PhaseRed red = reduce_pio2(theta);
     T s, c;
     sincos_poly((T)red.r, &s, &c);
-    T st, ct;
-    switch (red.q) {
-        case 0: st = s; ct = c; break;
-        case 1: st = c; ct = -s; break;
-        case 2: st = -s; ct = -c; break;
-        default: st = -c; ct = s; break;
-    }
+    // theta = r + q pi/2: (cos, sin) of theta is (c, s), (-s, c), (-c, -s),
+    // (s, -c) for q = 0..3; written as selects so that the lanes of a
+    // wavefront, whose quadrants differ, take no branch
+    const bool odd = red.q & 1;
+    T ct = odd ? s : c;
+    T st = odd ? c : s;
+    if ((red.q + 1) & 2) ct = -ct;
+    if (red.q & 2) st = -st;
     *re = ct;
     *im = -st;
 }

@@ -127,6 +127,10 @@ struct Ops {
     // rl: row stride, ps: plane stride, kv: valid columns per row
     static int spass(ifdm_ctx *c, void *arr, int rl, long ps, int kv, SArgs sa, int o0 = 0, int no = 0) {
         if (no == 0) no = N;
+        if ((MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1) && kv != N) {
+            c->err = "drift pass: every column of psi is transformed (kv = N)";
+            return IFDM_E_ARG;
+        }
         sa.other0 = o0;
         int rc;
         if ((rc = prof_begin(c))) return rc;

@@ -548,11 +548,13 @@ __device__ __forceinline__ void xcd_pair(int &bx, int &by) {
 // Minimum waves per SIMD asked of the compiler for a strided pass: 8 caps
 // the complex-float Green pass at 64 VGPRs, so two 1024-thread work-groups
 // share a CU (measured at 512^3: 0.39 -> 0.35 ms), for the sizes where two
-// such work-groups fit the LDS; 4 caps the narrow 512-thread tiles of
-// N = 1024 at 128 VGPRs, so two of them share a CU; the other passes fit
-// without a cap (capping the z-pass C spilled and slowed).
+// such work-groups fit the LDS, and the complex-float kick pass at N = 512
+// (68 VGPRs uncapped, one work-group per CU: 0.68 -> 1.02 ms; capped, 12
+// bytes of spill); 4 caps the narrow 512-thread tiles of N = 1024 at 128
+// VGPRs, so two of them share a CU; the other passes fit without a cap
+// (capping the z-pass C spilled and slowed).
 template <typename T, int MODE, int N, bool WIDE = false, int AXIS = 1> constexpr int sp_wpe() {
-    return (MODE == S_GREEN && sizeof(T) == 4 && N >= 128 && N <= 512) ? 8
+    return (sizeof(T) == 4 && ((MODE == S_GREEN && N >= 128 && N <= 512) || (MODE == S_KICK_DRIFT1 && N == 512))) ? 8
                                                                           : (s_narrow<T, N, WIDE, AXIS, MODE>() ? 4 : 1);
 }
 // The kick pass's phi loads.  Batched (0 < PB): each batch of PB loads is
@@ -562,9 +564,10 @@ template <typename T, int MODE, int N, bool WIDE = false, int AXIS = 1> constexp
 // (eight or sixteen memory round trips in series per tile).  Measured: FP64
 // A 1.12 -> 1.00 ms at 512^3; complex floats at 1024^3 with batches of 8
 // (of 16) 7.45 -> 6.70 ms (batches of 16: 6.92).  Complex floats up to
-// N = 512 keep the conditional loads (PB = 0): their pass holds 8 waves per
-// SIMD, which hide the round trips, and loads issued together take 78 VGPRs
-// (one work-group per CU: 0.71 -> 0.84 ms; capped at 64, they spill: 0.80).
+// N = 512 load one value per kick (PB = 0), unconditionally where every
+// column is valid (ALLCOLS in k_spass), and leave the schedule to the
+// compiler: batches of 8 took 78 VGPRs (one work-group per CU: 0.71 -> 0.84
+// ms; capped at 64, they spilled: 0.80).
 template <typename T, int N> constexpr int kick_pb() {
     return sizeof(T) == 8 ? SchedT<T, N>::E : (N == 1024 ? 8 : 0);
 }
@@ -602,14 +605,17 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
     } else if constexpr (MODE == S_KICK_DRIFT1 && (N / TC) % 16 == 0) {
         xcd_pair(bx, by);
     }
-    // 1024-point complex-float drift passes: the drift table in LDS.  From
-    // global memory its 16 loads per thread, issued after the forward
-    // transform under the 128-VGPR cap, were each awaited before the next
-    // (16 L2 round trips per tile).  Narrow tiles keep entries 0..N/2 (see
-    // spass_compute: two narrow work-groups per CU leave no room for all N).
+    // The drift passes at N = 512, and the complex-float ones at N = 1024,
+    // read the drift table from LDS.  From global memory its loads, issued
+    // after the forward transform, cost an L2 round trip per tile, and under
+    // the 128-VGPR cap of the 1024-point complex-float tiles each of the 16
+    // was awaited before the next.  Narrow tiles and complex doubles keep
+    // entries 0..N/2 (see spass_compute): two work-groups per CU leave no
+    // room for all N.  Measured at 512^3 (ms, global -> LDS table): mixed
+    // A 0.60 -> 0.54, B 0.48 -> 0.44; FP64 A 0.96 -> 0.95, B 0.88 -> 0.83.
     // The forward transform's exchanges order these writes before the reads.
-    constexpr bool LTAB = (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1) && N == 1024 && sizeof(T) == 4;
-    constexpr bool HALF = LTAB && s_narrow<T, N, WIDE, AXIS, MODE>();
+    constexpr bool LTAB = (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1) && (N == 512 || (N == 1024 && sizeof(T) == 4));
+    constexpr bool HALF = LTAB && (s_narrow<T, N, WIDE, AXIS, MODE>() || sizeof(T) == 8);
     constexpr int TABN = LTAB ? (HALF ? N / 2 + 1 : N) : 1;
     __shared__ cx<T> tabL[TABN];
     if constexpr (LTAB) {
@@ -617,14 +623,23 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
     }
     const int k0 = bx * TC, k = k0 + c;
     const int other = sa.other0 + block_order(sa.rev, by, gridDim.y);
-    const bool valid = k < kv;
+    constexpr bool PSI = MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1;
+    // The psi passes cover every column of every row (kv = N, checked at the
+    // launch), so where TC divides N no tile holds a column past the row.
+    // The kick pass then loads psi and phi without a condition: a load under
+    // a condition is a branch region of its own, awaited before the next one
+    // is issued (measured at 512^3: mixed A 0.68 -> 0.60 ms with the
+    // branch-free quadrant selection of expmi_parts, FP64 A 1.04 -> 0.96).
+    // The drift-only passes keep the condition (without it, mixed B took 74
+    // VGPRs, one work-group per CU: 0.47 -> 0.53 ms).
+    constexpr bool ALLCOLS = MODE == S_KICK_DRIFT1 && N % TC == 0;
+    const bool valid = ALLCOLS || k < kv;
     long base0, stride;
     spass_addr<N, AXIS>(other, k0, rl, ps, base0, stride);
     constexpr bool ROW128 = TC * sizeof(cx<T>) >= 128;
     // plain loads on the FP64 256-byte tiles and on rows narrower than a line
     const TileIO<T, N, ROW128 && !(WIDE && sizeof(T) == 8), ROW128> io(a, base0);
     cx<T> *L = lds + c * LS;
-    constexpr bool PSI = MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1;
     TwBase<T, N, tw_base_t<T, N, PSI>, SV> tb;
     if constexpr (acc_tw<T, N>() && PSI) load_twiddles<T, N>(tb, t, tw64);
     else load_twiddles<T, N>(tb, t, tw);
