@@ -69,6 +69,16 @@ struct ifdm_ctx {
     void *ck = nullptr;
     int ck_k = 0, ck_count = 0, ck_nsteps = -1;
     std::vector<ifdm_dt> ck_sched;   // the schedule the checkpoints were taken on
+    // the on-device step loop of ifdm_simulate_ex (option "device_loop"):
+    // its state, per-step records (IFDM_REC doubles each, up to
+    // IFDM_REC_CAP steps between host waits), the pieces of the quadrature's
+    // bisection, and a pinned copy of state and records
+    int dev_loop = 1;
+    bool dev_active = false;       // the launches being issued belong to the loop
+    void *dstep = nullptr;         // one ifdm::DevStep
+    double *drec = nullptr;
+    void *dpcs = nullptr;          // 3 x IFDM_QAGS_LIMIT ifdm::Piece
+    double *hrec = nullptr;        // pinned: DevStep, then the records
     // strided passes with persistent prefetching work-groups: bits 0, 1 the
     // y, x passes on psi; bits 2, 3, 4 the passes F, G, H on the half spectrum
     int persistent = 0;
@@ -85,6 +95,7 @@ struct ifdm_ctx {
 };
 
 static constexpr int IFDM_RED_BLOCKS = 256;
+static constexpr int IFDM_REC_CAP = 512;
 
 #define HIPCHK(ctx, expr)                                                                  \
     do {                                                                                   \

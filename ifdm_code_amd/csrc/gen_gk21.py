@@ -1,5 +1,5 @@
 """Computes the 21-point Gauss-Kronrod rule on [-1, 1] in 60-digit precision
-and prints it as the C++ table of the qk21 rule in cosmo.cpp.
+and prints it as the C++ tables of the qk21 rule in criteria.h.
 
 Gauss nodes: roots of the Legendre polynomial P_10.  Kronrod nodes: roots of
 the Stieltjes polynomial E_11, the monic odd polynomial of degree 11 with
@@ -74,16 +74,17 @@ dP = [i * c for i, c in enumerate(P10)][1:]
 gpos = [x for x in sorted(gauss, key=lambda z: -z) if x > 0]
 wg = [2 / ((1 - x * x) * sum(c * x ** i for i, c in enumerate(dP)) ** 2) for x in gpos]
 
-print("// 21-point Gauss-Kronrod rule on [-1,1], generated by gen_gk21.py")
-print("// xgk: positive Kronrod nodes, descending, then 0 (odd indices are the Gauss nodes)")
-print("static const double xgk[11] = {")
-for x in pos:
-    print("    " + mp.nstr(x, 25, min_fixed=-1, max_fixed=1) + ",")
-print("};\nstatic const double wgk[11] = {")
-for w in wk:
-    print("    " + mp.nstr(w, 25, min_fixed=-1, max_fixed=1) + ",")
-print("};\n// Gauss weights of the 10-point rule for the nodes xgk[1], xgk[3], .., xgk[9]")
-print("static const double wg[5] = {")
-for w in wg:
-    print("    " + mp.nstr(w, 25, min_fixed=-1, max_fixed=1) + ",")
-print("};")
+def macro(name, vals):
+    """a brace list of the values as one #define, three per line"""
+    items = [mp.nstr(v, 25, min_fixed=-1, max_fixed=1) for v in vals]
+    rows = [", ".join(items[i:i + 3]) for i in range(0, len(items), 3)]
+    print(f"#define IFDM_GK21_{name} \\")
+    print("    {" + ", \\\n     ".join(rows) + "}")
+
+
+print("// 21-point Gauss-Kronrod rule on [-1,1], generated by gen_gk21.py.")
+print("// Positive Kronrod nodes, descending, then 0 (odd indices are the Gauss nodes)")
+macro("XGK", pos)
+macro("WGK", wk)
+print("// Gauss weights of the 10-point rule for the nodes xgk[1], xgk[3], .., xgk[9]")
+macro("WG", wg)

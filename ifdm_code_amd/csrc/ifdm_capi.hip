@@ -65,6 +65,7 @@ struct Ops {
         z.rev = 0;
         z.ps = c->ps;
         z.sps = c->sps;
+        z.drhoc = nullptr;
         return z;
     }
     static SArgs sargs(ifdm_ctx *c, double dKE) {
@@ -81,6 +82,7 @@ struct Ops {
         s.aphi = 0.0;
         s.arho = 0.0;
         s.phi = c->phi;
+        s.dk = nullptr;
         return s;
     }
 
@@ -209,8 +211,13 @@ struct Ops {
             case 1: {
                 // 128-byte tiles here (two work-groups per CU in FP64 overlap the
                 // kick's sincos with loads: measured faster than 256-byte tiles)
-                if (aphi == 0.0 && arho == 0.0) return spass<1, S_DRIFT1>(c, c->psi, N, c->ps, N, sa);
                 SArgs sk = sa;
+                if (c->dev_active) {
+                    // the on-device loop: the coefficients the step preparation wrote
+                    sk.dk = &((const DevStep *)c->dstep)->kphi;
+                    return spass<1, S_KICK_DRIFT1>(c, c->psi, N, c->ps, N, sk);
+                }
+                if (aphi == 0.0 && arho == 0.0) return spass<1, S_DRIFT1>(c, c->psi, N, c->ps, N, sa);
                 sk.aphi = aphi;
                 sk.arho = arho * c->psc2;
                 return spass<1, S_KICK_DRIFT1>(c, c->psi, N, c->ps, N, sk);
@@ -221,6 +228,7 @@ struct Ops {
             case 3: {
                 ZArgs za = zargs(c, 0, 0);
                 za.dtab = c->dtab;
+                if (c->dev_active && c->mixed) za.drhoc = &((const DevStep *)c->dstep)->rho_mean;
                 c->rho_parts = c->nz_blocks;
                 return zpass<Z_DRIFT_SRC>(c, za, c->part_rho);
             }
@@ -234,9 +242,12 @@ struct Ops {
                 hipLaunchKernelGGL(k_reduce1, dim3(IFDM_RED_BLOCKS), dim3(256), 0, c->stream, c->part_rho,
                                    c->rho_parts, c->part_phi, c->nz_blocks, c->red_stage);
                 LAUNCHCHK(c);
-                hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, c->stream, c->red_stage, IFDM_RED_BLOCKS,
-                                   c->red);
-                LAUNCHCHK(c);
+                // (the on-device loop's next step preparation does the final stage)
+                if (!c->dev_active) {
+                    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, c->stream, c->red_stage, IFDM_RED_BLOCKS,
+                                       c->red);
+                    LAUNCHCHK(c);
+                }
                 return prof_end(c);
             }
             default: return IFDM_E_ARG;
@@ -350,6 +361,24 @@ struct Ops {
         return spass<0, S_INV, true>(c, c->psi, N, c->ps, N, sa);
     }
 
+    // one step of the on-device loop: the step preparation (as pass 0, in
+    // place of the drift table), the passes A..I with the device
+    // coefficients, and the first reduction stage; rec: the step's record
+    static int dev_step(ifdm_ctx *c, const StepIn &in, double *rec) {
+        c->cur_pass = 0;
+        int rc;
+        if ((rc = prof_begin(c))) return rc;
+        hipLaunchKernelGGL((k_step_prep<T, N>), dim3(1), dim3(256), 0, c->stream, (const double *)c->red_stage,
+                           IFDM_RED_BLOCKS, in, (DevStep *)c->dstep, rec, (cx<T> *)c->dtab, (const double *)c->k2,
+                           (Piece *)c->dpcs);
+        LAUNCHCHK(c);
+        if ((rc = prof_end(c))) return rc;
+        c->dev_active = true;
+        for (int i = 1; i < IFDM_NPASS && !rc; ++i) rc = pass(c, i, 0.0, 0.0, 0.0);
+        c->dev_active = false;
+        return rc;
+    }
+
     static int kick(ifdm_ctx *c, double aphi, double arho) {
         long nb = (c->np + 255) / 256;
         if (nb > 65536) nb = 65536;
@@ -370,12 +399,13 @@ struct Dispatch {
     int (*drift)(ifdm_ctx *, double);
     int (*fft3d)(ifdm_ctx *, int);
     int (*kick)(ifdm_ctx *, double, double);
+    int (*dev_step)(ifdm_ctx *, const StepIn &, double *);
 };
 
 template <typename T, int N>
 Dispatch make_dispatch() {
     return Dispatch{&Ops<T, N>::power_spectrum, &Ops<T, N>::adj_kick, &Ops<T, N>::theta_solve, &Ops<T, N>::pass, &Ops<T, N>::poisson, &Ops<T, N>::kick_drift_poisson, &Ops<T, N>::drift, &Ops<T, N>::fft3d,
-                    &Ops<T, N>::kick};
+                    &Ops<T, N>::kick, &Ops<T, N>::dev_step};
 }
 
 bool get_dispatch(int n, int precision, Dispatch *d) {
@@ -412,7 +442,7 @@ bool get_dispatch(int n, int precision, Dispatch *d) {
 // without the state they end in: they no longer serve a reverse sweep.
 inline void drop_checkpoints(ifdm_ctx *c) { c->ck_nsteps = -1; }
 
-int accumulate_profile(ifdm_ctx *c) {
+int accumulate_profile(ifdm_ctx *c, int steps = 1) {
     if (!c->prof) return 0;
     const size_t k = c->ev_pass.size();
     if (k == 0) return 0;
@@ -423,7 +453,7 @@ int accumulate_profile(ifdm_ctx *c) {
         c->pass_ms[c->ev_pass[i]] += ms;
     }
     c->ev_pass.clear();
-    c->pass_count++;
+    c->pass_count += steps;
     return 0;
 }
 
@@ -456,11 +486,12 @@ void release(ifdm_ctx *c) {
     for (void *b : icb)
         if (b) (void)hipFree(b);
     void *bufs[] = {c->psi, c->phi, c->shat, c->tw, c->k2, c->part_rho, c->part_phi, c->red, c->red_stage, c->dtab,
-                    c->adj_g, c->adj_s, c->adj_phs, c->ck};
+                    c->adj_g, c->adj_s, c->adj_phs, c->ck, c->dstep, c->drec, c->dpcs};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (c->tw64 && c->tw64 != c->tw) (void)hipFree(c->tw64);
     if (c->red_host) (void)hipHostFree(c->red_host);
+    if (c->hrec) (void)hipHostFree(c->hrec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -557,12 +588,16 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
         {(void **)&c->red, 3 * ds},
         {(void **)&c->red_stage, 3 * ds * IFDM_RED_BLOCKS},
         {&c->dtab, cs * (size_t)n},
+        {&c->dstep, sizeof(DevStep)},
+        {(void **)&c->drec, ds * IFDM_REC * (size_t)IFDM_REC_CAP},
+        {&c->dpcs, sizeof(Piece) * 3 * (size_t)IFDM_QAGS_LIMIT},
     };
     for (auto &a : allocs) {
         if ((e = hipMalloc(a.p, a.bytes)) != hipSuccess)
             return fail(IFDM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     }
-    if ((e = hipHostMalloc((void **)&c->red_host, 3 * ds)) != hipSuccess)
+    if ((e = hipHostMalloc((void **)&c->red_host, 3 * ds)) != hipSuccess ||
+        (e = hipHostMalloc((void **)&c->hrec, sizeof(DevStep) + ds * IFDM_REC * (size_t)IFDM_REC_CAP)) != hipSuccess)
         return fail(IFDM_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
     // twiddles W_N^k = exp(-2 pi i k / N) evaluated in long double
     std::vector<double> tw, k2(n);
@@ -809,7 +844,66 @@ int ifdm_simulate_ex(ifdm_ctx *c, double a0, int nsteps, double *a_out, ifdm_dt 
         return rc;                                               // :218 of the first step
     }
     double a = a0;
-    for (int s = 0; s < nsteps; ++s) {
+    if (c->dev_loop && nsteps > 0) {
+        // the on-device loop (k_step_prep): the same steps as the host loop
+        // below, bit for bit, with no host wait between steps
+        Dispatch d;
+        get_dispatch(c->n, c->cfg.precision, &d);
+        HIPCHK(c, hipSetDevice(c->cfg.device));
+        drop_checkpoints(c);
+        StepIn in{};
+        in.cc = crit_const(&c->cfg);
+        in.a0 = a;
+        in.max_phi = st.max_phi;
+        in.max_rho = st.max_rho;
+        in.norm = st.norm;
+        in.has_pend = c->pending;
+        in.pend_aphi = c->pend_aphi;
+        in.pend_arho = c->pend_arho;
+        in.psc2 = c->psc2;
+        in.ms = c->ms;
+        in.n3 = (double)c->n3;
+        in.m_over_hbar = c->m_over_hbar;
+        in.g_over_hbar = c->g_over_hbar;
+        in.ke_coef = c->ke_coef;
+        c->run_stats_valid = false;
+        // steps between host waits: the record buffer, and with live
+        // profiling the event pool (every pass of every step)
+        int chunk = IFDM_REC_CAP;
+        if (c->prof) chunk = std::min(chunk, (IFDM_EV_POOL / 2 - 1) / IFDM_NPASS);
+        const double *rh = c->hrec + sizeof(DevStep) / sizeof(double);
+        for (int s0 = 0; s0 < nsteps; s0 += chunk) {
+            const int ns = std::min(chunk, nsteps - s0);
+            for (int s = 0; s < ns; ++s) {
+                in.first = s0 + s == 0;
+                if ((rc = d.dev_step(c, in, c->drec + IFDM_REC * s))) return rc;
+            }
+            c->pending = true;
+            HIPCHK(c, hipMemcpyAsync(c->hrec, c->dstep, sizeof(DevStep), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipMemcpyAsync((void *)rh, c->drec, sizeof(double) * IFDM_REC * ns, hipMemcpyDeviceToHost,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            for (int s = 0; s < ns; ++s) {
+                if (sched_out) {
+                    sched_out[s0 + s].dKE = rh[IFDM_REC * s];
+                    sched_out[s0 + s].dPE = rh[IFDM_REC * s + 1];
+                    sched_out[s0 + s].dQI = rh[IFDM_REC * s + 2];
+                }
+                a = rh[IFDM_REC * s + 3];
+            }
+            if ((rc = accumulate_profile(c, ns))) return rc;
+        }
+        const DevStep ds = *(const DevStep *)c->hrec;
+        c->pend_aphi = ds.pend_aphi;
+        c->pend_arho = ds.pend_arho;
+        // the last step's reductions, as ifdm_step reads them
+        hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, c->stream, c->red_stage, IFDM_RED_BLOCKS, c->red);
+        LAUNCHCHK(c);
+        if ((rc = read_stats(c, &st))) return rc;
+        c->run_stats = st;
+        c->run_stats_valid = true;
+    }
+    for (int s = 0; s < nsteps && !c->dev_loop; ++s) {
         ifdm_dt dt;
         double an;
         if ((rc = ifdm_timestep_criteria(&c->cfg, a, &st, &dt, &an, nullptr))) {
@@ -1174,6 +1268,11 @@ int ifdm_set_option(ifdm_ctx *c, const char *name, long value) {
             return IFDM_E_ARG;
         }
         c->prof_mask = (unsigned)value;
+        return IFDM_OK;
+    }
+    if (std::strcmp(name, "device_loop") == 0) {
+        if (value != 0 && value != 1) { c->err = "device_loop must be 0 (time step on the host) or 1 (on the device)"; return IFDM_E_ARG; }
+        c->dev_loop = (int)value;
         return IFDM_OK;
     }
     if (std::strcmp(name, "ic_deposit") == 0) {

@@ -25,6 +25,7 @@
 #pragma once
 #include <type_traits>
 
+#include "criteria.h"
 #include "fastmath.h"
 #include "fft_core.h"
 
@@ -47,6 +48,7 @@ struct ZArgs {
     int rev;          // blocks walk the lines in descending order
     long ps;          // plane stride (elements) of psi and phi: N*N + padding
     long sps;         // plane stride (elements) of shat: N*HS + padding
+    const double *drhoc;  // when set: rhoc from device memory (the on-device step loop)
 };
 
 struct SArgs {
@@ -64,6 +66,7 @@ struct SArgs {
     double aphi;
     double arho;
     const void *phi;
+    const double *dk;  // when set: aphi, arho from device memory (the on-device step loop)
 };
 
 // Block order.  A pass that walks its lines opposite to the pass before it
@@ -343,12 +346,13 @@ void k_zpass(cx<T> *__restrict__ psi, T *__restrict__ phi, cx<T> *__restrict__ s
         }
         if constexpr (MODE == Z_SRC || MODE == Z_DRIFT_SRC) {
             // the source replaces psi in the registers (psi is stored already)
+            const double rhoc = za.drhoc ? *za.drhoc : za.rhoc;
             double mx = 0.0, sm = 0.0;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
                 if (valid) { mx = fmax(mx, rr); sm += rr; }
-                v[e] = mkc<T>(valid ? (T)(za.ms * rr - za.rhoc) : (T)0, (T)0);
+                v[e] = mkc<T>(valid ? (T)(za.ms * rr - rhoc) : (T)0, (T)0);
             }
             fft_line<T, N, -1, wave_line<T, N>(), false, false>(v, L, t, tb);
             if (valid) {
@@ -542,6 +546,49 @@ template <typename T, int N, bool NTL = false, bool NTS = true> struct TileIO {
     }
 };
 
+// Element e (row t + e P) of this lane's column of a tile: a wave-uniform
+// address per e (the tile's row 0 plus e P rows, formed in scalar
+// registers) plus one per-lane 32-bit byte offset (column c, row t) shared
+// by every e.  It serves the kick pass, whose registers are the scarcest:
+// there a 64-bit address per point (N = 1024, where whole-array offsets
+// exceed 32 bits; and phi's at every N), held from the loads to the stores,
+// took 2 VGPRs per point and spilled (12-96 bytes per lane; the spill
+// traffic was 8 % of the pass's bytes at 512^3 and 23-40 % at 1024^3).
+// The lane offset is at most (P - 1) rows of the largest stride, under
+// 2.2 GB for every pass here.
+
+// a wave-uniform 64-bit value moved to scalar registers
+__device__ __forceinline__ long uniform64(long v) {
+    return (long)__builtin_amdgcn_readfirstlane((int)(v >> 32)) * (1L << 32) +
+           (unsigned)__builtin_amdgcn_readfirstlane((int)(v & 0xffffffffL));
+}
+
+template <typename V, bool NTL = false, bool NTS = false> struct ColIO {
+    const char *a0;     // the array (a kernel argument: global memory)
+    long ub;            // wave-uniform: byte offset of the tile's row 0
+    long step;          // wave-uniform: bytes from row t + e P to row t + (e + 1) P
+    unsigned off;       // per lane
+    __device__ __forceinline__ ColIO(const V *a, long uniform_elem, long stride, int rows, long lane_elem) {
+        a0 = reinterpret_cast<const char *>(a);
+        ub = uniform64(uniform_elem * (long)sizeof(V));
+        step = uniform64((long)rows * stride * (long)sizeof(V));
+        off = (unsigned)lane_elem * (unsigned)sizeof(V);
+    }
+    __device__ __forceinline__ V *ptr(int e) const {
+        // the row's offset formed in scalar registers; the lane adds only
+        // its 32-bit offset
+        return reinterpret_cast<V *>(const_cast<char *>(a0) + uniform64(ub + e * step) + off);
+    }
+    __device__ __forceinline__ V load(int e) const {
+        if constexpr (NTL) return stream_load(ptr(e));
+        else return *ptr(e);
+    }
+    __device__ __forceinline__ void store(int e, V v) const {
+        if constexpr (NTS) stream_store(ptr(e), v);
+        else *ptr(e) = v;
+    }
+};
+
 // Work-groups start round-robin over the 8 XCDs (linear id mod 8), each
 // with its own L2.  The kick pass's column tiles 2j and 2j+1 each read half
 // of every 128-byte segment of phi; this maps them to linear ids 8 apart,
@@ -649,39 +696,48 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
     spass_addr<N, AXIS>(other, k0, rl, ps, base0, stride);
     constexpr bool ROW128 = TC * sizeof(cx<T>) >= 128;
     // plain loads on the FP64 256-byte tiles and on rows narrower than a line
-    const TileIO<T, N, ROW128 && !(WIDE && sizeof(T) == 8), ROW128> io(a, base0);
+    constexpr bool NTL = ROW128 && !(WIDE && sizeof(T) == 8);
+    const TileIO<T, N, NTL, ROW128> io(a, base0);
+    // the kick pass: per-e uniform addresses and one lane offset (ColIO;
+    // measured at 1024^3, mixed A 6.25 -> 4.93 ms, FP64 A 12.1 -> 9.6 ms; the
+    // other passes, which do not spill, the same either way)
+    constexpr bool COL = MODE == S_KICK_DRIFT1;
+    const ColIO<cx<T>, NTL, ROW128> cio(a, base0, stride, P, c + (long)t * stride);
     cx<T> *L = lds + c * LS;
     TwBase<T, N, tw_base_t<T, N, PSI>, SV> tb;
     if constexpr (acc_tw<T, N>() && PSI) load_twiddles<T, N>(tb, t, tw64);
     else load_twiddles<T, N>(tb, t, tw);
     cx<T> v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = valid ? io.load(c + (long)(t + e * P) * stride) : mkc<T>(0, 0);
+    for (int e = 0; e < E; ++e)
+        v[e] = valid ? (COL ? cio.load(e) : io.load(c + (long)(t + e * P) * stride)) : mkc<T>(0, 0);
     if constexpr (MODE == S_KICK_DRIFT1) {
+        const double kaphi = sa.dk ? sa.dk[0] : sa.aphi, karho = sa.dk ? sa.dk[1] : sa.arho;
         // phi loads: see kick_pb.  Batched, a column past kv reads the
         // tile's first column, which lies inside the row (k0 < kv), and its
         // value is not used.
-        const T *ph = (const T *)sa.phi + base0;
         constexpr int PB = kick_pb<T, N>();
+        const int cp = valid ? c : 0;
+        // phi through ColIO as psi
+        const ColIO<T> pio((const T *)sa.phi, base0, stride, P, cp + (long)t * stride);
         if constexpr (PB == 0) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                const double f = valid ? (double)ph[c + (long)(t + e * P) * stride] : 0.0;
+                const double f = valid ? (double)pio.load(e) : 0.0;
                 const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
-                v[e] = cmul(v[e], expmi<T>(sa.aphi * f + sa.arho * rr));
+                v[e] = cmul(v[e], expmi<T>(kaphi * f + karho * rr));
             }
         } else {
-            const int cp = valid ? c : 0;
 #pragma unroll
             for (int e0 = 0; e0 < E; e0 += PB) {
                 T f[PB];
 #pragma unroll
-                for (int i = 0; i < PB; ++i) f[i] = ph[cp + (long)(t + (e0 + i) * P) * stride];
+                for (int i = 0; i < PB; ++i) f[i] = pio.load(e0 + i);
 #pragma unroll
                 for (int i = 0; i < PB; ++i) {
                     const int e = e0 + i;
                     const double rr = (double)v[e].x * v[e].x + (double)v[e].y * v[e].y;
-                    v[e] = cmul(v[e], expmi<T>(sa.aphi * (valid ? (double)f[i] : 0.0) + sa.arho * rr));
+                    v[e] = cmul(v[e], expmi<T>(kaphi * (valid ? (double)f[i] : 0.0) + karho * rr));
                 }
                 if (PB < E) __builtin_amdgcn_sched_barrier(0);
             }
@@ -691,7 +747,10 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
                                                                         LTAB ? (const cx<T> *)tabL : (const cx<T> *)sa.dtab, sa.k2);
     if (valid) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) io.store(c + (long)(t + e * P) * stride, v[e]);
+        for (int e = 0; e < E; ++e) {
+            if constexpr (COL) cio.store(e, v[e]);
+            else io.store(c + (long)(t + e * P) * stride, v[e]);
+        }
     }
 }
 
@@ -925,6 +984,114 @@ __global__ __launch_bounds__(256) void k_reduce2(const double *__restrict__ stag
     const double b = block_reduce<true>(mr, sb, tid, 256);
     const double c = block_reduce<false>(sm, sb, tid, 256);
     if (tid == 0) { out[0] = a; out[1] = b; out[2] = c; }
+}
+
+// ------------------------------------------------------ on-device step loop
+// The adaptive loop of Simulation (src/CosmoBEC_MPI_stan.cpp:212-304) with
+// the time step decided on the device: after each step's passes, one
+// work-group reduces the partials, evaluates Timestep_criteria (criteria.h,
+// the same arithmetic as the host's ifdm_timestep_criteria), advances a,
+// forms the kick coefficients of the next kick pass (the pending second
+// half-kick plus the new first one) and writes the drift table.  The host
+// then issues every step's launches back to back with no wait between
+// steps (a wait, the copy of the reductions and the host's criteria took
+// 30-40 us of idle GPU per step).
+
+// loop state in device memory
+struct DevStep {
+    double a;                       // scale factor after the last prepared step
+    double kphi, krho;              // kick coefficients of the next kick pass (krho times psc2)
+    double pend_aphi, pend_arho;    // second half-kick of the last prepared step (pending)
+    double rho_mean;                // m_s <|psi|^2> of the reductions read (mixed-mode source)
+    double max_phi, max_rho, norm;  // the reductions read
+};
+// what a preparation takes from the host
+struct StepIn {
+    CritConst cc;
+    double a0;                       // first step of a call: the scale factor
+    double max_phi, max_rho, norm;   // first step: the reductions (host values)
+    double pend_aphi, pend_arho;     // first step: a half-kick pending from before
+    double psc2, ms, n3, m_over_hbar, g_over_hbar, ke_coef;
+    int first, has_pend;
+};
+// per-step record: dKE, dPE, dQI, a after the step
+static constexpr int IFDM_REC = 4;
+static constexpr int IFDM_QAGS_LIMIT = 1000;
+
+template <typename T, int N>
+__global__ __launch_bounds__(256) void k_step_prep(const double *__restrict__ stage, int nst, StepIn in,
+                                                   DevStep *__restrict__ st, double *__restrict__ rec,
+                                                   cx<T> *__restrict__ tab, const double *__restrict__ k2,
+                                                   Piece *__restrict__ pcs) {
+    IFDM_FP_EXACT
+    __shared__ double sb[16];
+    __shared__ double fv[3][21];
+    __shared__ double sh[5];   // a, a + da, dKE, dPE, dQI
+    const int tid = threadIdx.x;
+    double max_phi, max_rho, norm;
+    if (in.first) {
+        max_phi = in.max_phi;
+        max_rho = in.max_rho;
+        norm = in.norm;
+    } else {
+        // the final reduction of k_reduce2, then read_stats's scaling
+        double mp = 0.0, mr = 0.0, sm = 0.0;
+        for (int i = tid; i < nst; i += 256) {
+            mp = fmax(mp, stage[3 * i]);
+            mr = fmax(mr, stage[3 * i + 1]);
+            sm += stage[3 * i + 2];
+        }
+        max_phi = block_reduce<true>(mp, sb, tid, 256);
+        max_rho = block_reduce<true>(mr, sb, tid, 256) * in.psc2;
+        norm = block_reduce<false>(sm, sb, tid, 256) * in.psc2;
+    }
+    if (tid == 0) {
+        const double a = in.first ? in.a0 : st->a;
+        sh[0] = a;
+        sh[1] = a + criteria_da(in.cc, a, max_phi, max_rho, nullptr);
+    }
+    __syncthreads();
+    const double a = sh[0], b = sh[1];
+    // dKE, dPE, dQI: the integrals of 1/(H a^n), n = 3, 2, 4 (:66-68)
+    auto power = [](int i) { return i == 0 ? 3 : (i == 1 ? 2 : 4); };
+    if (tid < 63) fv[tid / 21][tid % 21] = integrand(in.cc, gk21_node(a, b, tid % 21), power(tid / 21));
+    __syncthreads();
+    if (tid < 3) {
+        double err, rabs, rasc;
+        double r = qk21_values(a, b, fv[tid], &err, &rabs, &rasc);
+        if (!qags_first_accepted(r, err, rasc, 1e-10)) {
+            bool ok;
+            r = qags_bisect(in.cc, a, b, power(tid), r, err, 1e-10, IFDM_QAGS_LIMIT, pcs + IFDM_QAGS_LIMIT * tid, &ok);
+        }
+        sh[2 + tid] = r;
+    }
+    __syncthreads();
+    const double dKE = sh[2];
+    if (tid == 0) {
+        const double dPE = sh[3], dQI = sh[4];
+        // ifdm_step's coefficients: the half-kick of this step plus the pending one
+        const double aphi = 0.5 * dPE * in.m_over_hbar, arho = 0.5 * dQI * in.g_over_hbar;
+        const bool pend = in.first ? in.has_pend : true;
+        double kphi = aphi, krho = arho;
+        if (pend) {
+            kphi += in.first ? in.pend_aphi : st->pend_aphi;
+            krho += in.first ? in.pend_arho : st->pend_arho;
+        }
+        st->a = b;
+        st->kphi = kphi;
+        st->krho = krho * in.psc2;
+        st->pend_aphi = aphi;
+        st->pend_arho = arho;
+        st->rho_mean = in.ms * norm / in.n3;
+        st->max_phi = max_phi;
+        st->max_rho = max_rho;
+        st->norm = norm;
+        rec[0] = dKE;
+        rec[1] = dPE;
+        rec[2] = dQI;
+        rec[3] = b;
+    }
+    write_drift_table<T, N>(tab, k2, in.ke_coef * dKE, tid, 256);
 }
 
 }  // namespace ifdm

@@ -128,10 +128,15 @@ int ifdm_run(ifdm_ctx *ctx, const ifdm_dt *sched, int nsteps, ifdm_stats *stats_
 
 /* The reference's time loop (Simulation, src/CosmoBEC_MPI_stan.cpp:212-304)
  * without its snapshot triggers: a Poisson solve, then per step
- * Timestep_criteria (host, FP64) and Split_op_MPI, with the second half-kick
- * of each step merged into the first of the next.  a0: scale factor at the
+ * Timestep_criteria (FP64) and Split_op_MPI, with the second half-kick of
+ * each step merged into the first of the next.  a0: scale factor at the
  * start; a_out (may be NULL): scale factor after the last step; sched_out
- * (nsteps entries, may be NULL): the integrals used, for replay. */
+ * (nsteps entries, may be NULL): the integrals used, for replay.  By default
+ * (option "device_loop" 1) the time step is decided on the device between
+ * the steps, with the arithmetic of ifdm_timestep_criteria, so the host
+ * issues every step without waiting; the schedule and the states equal the
+ * host-driven loop (device_loop 0: ifdm_timestep_criteria and ifdm_step
+ * per step) bit for bit. */
 int ifdm_simulate(ifdm_ctx *ctx, double a0, int nsteps, double *a_out, ifdm_dt *sched_out);
 
 /* ifdm_simulate with flags, so one run can be driven in pieces (to time a
@@ -147,7 +152,9 @@ int ifdm_simulate(ifdm_ctx *ctx, double a0, int nsteps, double *a_out, ifdm_dt *
 int ifdm_simulate_ex(ifdm_ctx *ctx, double a0, int nsteps, double *a_out, ifdm_dt *sched_out, int flags);
 
 /* Times each pass of one step with HIP events on the context's stream,
- * averaged over iters repetitions (ms_out[IFDM_NPASS]: the drift table; the
+ * averaged over iters repetitions (ms_out[IFDM_NPASS]: the drift table (in
+ * the on-device loop of ifdm_simulate, the step preparation: the final
+ * reduction, the time step and the drift table); the
  * passes A, B, C - the y, x, z drift round trips, A with the kick, C with
  * the Poisson source and its z transform; F, G, H, I - the Poisson solve on
  * the half spectrum; then the final reduction).  The state is advanced. */
@@ -169,7 +176,10 @@ int ifdm_time_passes(ifdm_ctx *ctx, const ifdm_dt *dt, int iters, float *ms_out)
  *   its inverse y pass (H); a set bit launches a resident set of
  *   work-groups that walk the tiles and prefetch the next tile's data during
  *   the current transform instead of one work-group per tile.  Default: 8
- *   (G) in FP64, 0 in mixed. */
+ *   (G) in FP64, 0 in mixed.
+ *   "device_loop": 1 (default) ifdm_simulate / ifdm_simulate_ex decide the
+ *   time step on the device; 0 on the host after each step (the same
+ *   results; a host wait per step). */
 int ifdm_set_option(ifdm_ctx *ctx, const char *name, long value);
 
 /* Live per-pass timing: with on != 0 every step run by ifdm_run or

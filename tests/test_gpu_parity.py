@@ -558,3 +558,40 @@ def test_mixed_set_psi_device_refreshes_mean_density():
         hip.hipFree(dev)
     assert st_dev["norm"] == pytest.approx(st_host["norm"], rel=1e-12)
     assert rel_l2(phi_dev, phi_host) < 1e-7
+
+
+@pytest.mark.parametrize("prec,n,steps", [(0, 16, 600), (1, 16, 600), (0, 64, 30), (1, 64, 30)])
+def test_device_loop_equals_host_loop(prec, n, steps):
+    """ifdm_simulate_ex decides each time step on the device (option
+    "device_loop" 1, the default: the step preparation kernel reduces the
+    partials, evaluates the criteria and the three integrals with the
+    host's arithmetic, and writes the kick coefficients and the drift
+    table).  Against the host loop (device_loop 0: ifdm_timestep_criteria
+    and ifdm_step per step): the same schedule, scale factor and state bit
+    for bit, over more steps than one batch of records (512) at 16^3, with
+    the run split in pieces as bench.py drives it."""
+    F, _O = _mods()
+    cfg = F.Config.from_cli(2.5e-24, G_IN, 5.0, n, precision=prec)
+    psi = make_psi(n, 21, cfg.mfdm_solar)
+    out = {}
+    for loop in (0, 1):
+        _say(f"[device loop] prec={prec} n={n} steps={steps} device_loop={loop}")
+        with F.Solver(cfg) as s:
+            s.set_option("device_loop", loop)
+            s.set_psi(psi)
+            a, sched = s.simulate_native(1.0 / 51, steps // 3, keep_pending=True)
+            a, d = s.simulate_native(a, steps - steps // 3, resume=True)
+            out[loop] = (a, sched + d, s.get_psi(), s.get_phi())
+    a0, s0, p0, f0 = out[0]
+    a1, s1, p1, f1 = out[1]
+    assert a1 == a0
+    assert s1 == s0
+    assert np.array_equal(p1, p0)
+    assert np.array_equal(f1, f0)
+
+
+def test_device_loop_option_is_checked():
+    F, _O = _mods()
+    with F.Solver(_cfg(F, 8)) as s:
+        with pytest.raises(F.IfdmError, match="device_loop"):
+            s.set_option("device_loop", 2)

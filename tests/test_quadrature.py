@@ -11,10 +11,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def gk_table():
-    src = open(os.path.join(ROOT, "ifdm_code_amd", "csrc", "cosmo.cpp")).read()
+    src = open(os.path.join(ROOT, "ifdm_code_amd", "csrc", "criteria.h")).read()
     out = {}
     for name in ("xgk", "wgk", "wg"):
-        body = re.search(r"static const double %s\[\d+\] = \{(.*?)\};" % name, src, re.S).group(1)
+        body = re.search(r"#define IFDM_GK21_%s\s*\\\s*\{(.*?)\}" % name.upper(), src, re.S).group(1)
         out[name] = np.array([float(v) for v in re.findall(r"[-+0-9.eE]+", body)])
     return out
 
