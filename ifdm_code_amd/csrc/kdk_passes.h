@@ -105,21 +105,26 @@ template <typename T, int N> constexpr int z_lines() {
     constexpr int P = N / SchedT<T, N>::E;
     return (256 / P) < (N * N) ? (256 / P) : (N * N);
 }
-// N = 1024 y passes (AXIS 1): 64-byte row segments ("narrow" tiles), so a
+// N = 1024 strided passes: 64-byte row segments ("narrow" tiles), so a
 // tile's exchange buffer takes 74 KB of LDS and two work-groups share a CU,
 // one loading while the other transforms (a 128-byte tile of 1024-point
 // lines needs 149 KB: one work-group per CU, whose loads and transforms
 // never overlap).  The two tiles of a 128-byte line run on one XCD at about
-// the same time (xcd_pair_rows), so the second finds the line in L2.  The
-// x passes (AXIS 0) keep 128-byte rows: their rows lie in N different
-// x-planes, one address translation each, and narrow rows would move half
-// the bytes per translation (measured at 1024^3: B 5.8 -> 6.9 ms mixed,
-// 9.9 -> 11.3 ms FP64).  Measured at 1024^3 with the pairing (without):
-// A 8.03 -> 7.59 (8.51) ms mixed, 13.09 -> 12.34 (14.07) ms FP64; the
-// half-spectrum y passes F, H 1.88, 1.91 (1.97, 2.01) ms mixed; in FP64 they
-// keep 128-byte rows (3.57, 3.46 against 3.64, 3.65 ms narrow).
+// the same time (xcd_group_rows), so the second finds the line in L2.
+// Measured at 1024^3 with the grouping (without): A 8.03 -> 7.59 (8.51) ms
+// mixed, 13.09 -> 12.34 (14.07) ms FP64; the half-spectrum y passes F, H
+// 1.88, 1.91 (1.97, 2.01) ms mixed; in FP64 they keep 128-byte rows (3.57,
+// 3.46 against 3.64, 3.65 ms narrow).  The x passes B and G take narrow
+// tiles too, grouped the same way (B 5.40 -> 4.98 ms mixed, 10.3 -> 9.6 ms
+// FP64; G 2.98 -> 2.63 ms mixed; the FP64 G pass keeps 128-byte rows: 5.67
+// ms either way, and narrow rows wrote 12 % more than its bytes); without
+// the grouping and the addressing of the current build they had measured
+// slower (B 5.8 -> 6.9 ms mixed).  At N = 512, where two 128-byte tiles
+// already share a CU, narrow x tiles are slower (FP64 B 0.83 -> 1.05 ms,
+// mixed B 0.44 -> 0.46, G 0.248 -> 0.256).
 template <typename T, int N, bool WIDE = false, int AXIS = 1, int MODE = -1> constexpr bool s_narrow() {
-    return N == 1024 && !WIDE && AXIS == 1 && (sizeof(T) == 4 || MODE == S_KICK_DRIFT1 || MODE == S_DRIFT1);
+    return N == 1024 && !WIDE && ((AXIS == 1 && (sizeof(T) == 4 || MODE == S_KICK_DRIFT1 || MODE == S_DRIFT1)) ||
+                                  (AXIS == 0 && (MODE == S_DRIFT1 || (MODE == S_GREEN && sizeof(T) == 4))));
 }
 
 // Schedule variant of a strided pass's transforms (Sched): radix-16 stages
