@@ -595,3 +595,35 @@ def test_device_loop_option_is_checked():
     with F.Solver(_cfg(F, 8)) as s:
         with pytest.raises(F.IfdmError, match="device_loop"):
             s.set_option("device_loop", 2)
+
+
+@pytest.mark.parametrize("loop", [0, 1])
+def test_live_profiling_counts_every_pass(loop):
+    """Live per-pass timing (ifdm_set_profiling): every pass of every step
+    carries a pair of HIP events (single-launch passes attach them to the
+    launch itself), the profile counts the steps, and a profile mask keeps
+    only the chosen pass; the state is the unprofiled run's bit for bit."""
+    F, _O = _mods()
+    n = 64
+    cfg = F.Config.from_cli(2.5e-24, G_IN, 5.0, n)
+    psi = make_psi(n, 3, cfg.mfdm_solar)
+    with F.Solver(cfg) as s:
+        s.set_option("device_loop", loop)
+        s.set_psi(psi)
+        s.simulate_native(1.0 / 51, 6)
+        ref = s.get_psi()
+    with F.Solver(cfg) as s:
+        s.set_option("device_loop", loop)
+        s.set_psi(psi)
+        s.set_profiling(True)
+        a, _ = s.simulate_native(1.0 / 51, 3, keep_pending=True)
+        ms, cnt = s.get_profile()
+        assert cnt == 3
+        assert all(x > 0.0 for x in ms), ms
+        s.set_option("profile_mask", 1 << 3)
+        s.set_profiling(True)
+        s.simulate_native(a, 3, resume=True)
+        ms2, cnt2 = s.get_profile()
+        assert cnt2 == 3 and ms2[3] > 0.0
+        assert all(x == 0.0 for i, x in enumerate(ms2) if i != 3), ms2
+        assert np.array_equal(s.get_psi(), ref)
