@@ -72,7 +72,6 @@ struct Ops {
         SArgs s;
         s.other0 = 0;
         s.dtab = c->dtab;
-        s.k2 = c->k2;
         s.ke_coef = c->ke_coef;
         s.dKE = dKE;
         s.green = c->green;
@@ -556,8 +555,10 @@ int ifdm_create(const ifdm_config *cfg, ifdm_ctx **out) {
     // (the half spectrum's plane stride N*HS is no power of two: padding it
     // measured neutral)
     c->sps = (long)cfg->n * c->hs;
-    // persistent work-groups measured faster on the FP64 passes G and H only
-    c->persistent = c->mixed ? 0 : 8;
+    // one work-group per tile on every strided pass: persistent work-groups
+    // (option "persistent") measured slower or equal at 512^3 (FP64 G 0.459
+    // against 0.463 ms, H 0.385 against 0.408)
+    c->persistent = 0;
     const int n = c->n;
     auto fail = [&](int code, const std::string &m) {
         g_create_err = m;

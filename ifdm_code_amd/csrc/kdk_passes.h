@@ -54,7 +54,6 @@ struct ZArgs {
 struct SArgs {
     int other0;        // first line index of the fixed axis in this launch
     const void *dtab;  // drift table E[i] (passes A, B)
-    const double *k2;  // k_i^2, i < N (src/struct3D.h:76-85 squared)
     double ke_coef;    // 0.5 hbar / (m / c^2)   (src/struct3D.h:121)
     double dKE;        // time integral of the drift
     double green;      // -4 pi G
@@ -456,14 +455,14 @@ void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T>
 
 // the transform (and the spectral multiply of passes C and G) of one tile;
 // FRESH: the first transform of the work-group (no earlier LDS use).  tab
-// and k2 are the drift table and k_i^2 (global memory, or LDS copies).
+// is the drift table (global memory, or an LDS copy).
 // HALF: tab holds the drift table for i <= N/2 only, and point i takes entry
 // min(i, N - i) (k^2 of i and N - i are the same double: kvec(i) for i >= N/2
 // is the exact negation of kvec(N - i), so the table entries are equal).
 template <typename T, int N, int MODE, bool FRESH, int AXIS, typename W, int V, bool HALF = false>
 __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const TwBase<T, N, W, V> &tw,
                                               const SArgs &sa, int other, int k, bool valid,
-                                              const cx<T> *tab, const double *k2) {
+                                              const cx<T> *tab) {
     constexpr int E = SchedT<T, N>::E, P = N / E;
     constexpr bool ACC = acc_tw<T, N>() && (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1);
     if constexpr (MODE == S_FWD) fft_line<T, N, -1, false, FRESH, ACC>(v, L, t, tw);
@@ -484,8 +483,8 @@ __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const T
                 // from the integer |m|^2 = m_x^2 + m_y^2 + m_z^2 (exact), no
                 // table loads: k^2 = (2 pi / L)^2 |m|^2
                 const float g = (float)(sa.green * sa.fftnorm / sa.kunit2);
-                const int mo = other < N / 2 ? other : other - N;
-                const int mok = mo * mo + (valid ? k * k : 0);     // k <= N/2 on the half spectrum
+                const int mo = other < N / 2 ? other : other - N, mk = k < N / 2 ? k : k - N;
+                const int mok = mo * mo + (valid ? mk * mk : 0);
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     const int i = t + e * P, mi = i < N / 2 ? i : i - N;
@@ -493,12 +492,17 @@ __device__ __forceinline__ void spass_compute(cx<T> *v, cx<T> *L, int t, const T
                     v[e] = cscale(v[e], m2 == 0 ? 0.f : g * __builtin_amdgcn_rcpf((float)m2));
                 }
             } else {
-                const double k2o = k2[other], k2k = valid ? k2[k] : 0.0;
+                // complex doubles: the same integer |m|^2, one FP64 quotient
+                // per point, no table loads (k runs over the half spectrum,
+                // or over all N columns in the initial conditions' solve)
+                const double g = sa.green * sa.fftnorm / sa.kunit2;
+                const int mo = other < N / 2 ? other : other - N, mk = k < N / 2 ? k : k - N;
+                const int mok = mo * mo + (valid ? mk * mk : 0);
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
-                    const double kk = k2[t + e * P] + k2o + k2k;
-                    const double gf = (kk == 0.0) ? 0.0 : sa.green / kk * sa.fftnorm;
-                    v[e] = cscale(v[e], (T)gf);
+                    const int i = t + e * P, mi = i < N / 2 ? i : i - N;
+                    const int m2 = mi * mi + mok;
+                    v[e] = cscale(v[e], m2 == 0 ? 0.0 : g / (double)m2);
                 }
             }
         }
@@ -749,7 +753,7 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
         }
     }
     spass_compute<T, N, MODE, true, AXIS, tw_base_t<T, N, PSI>, SV, HALF>(v, L, t, tb, sa, other, k, valid,
-                                                                        LTAB ? (const cx<T> *)tabL : (const cx<T> *)sa.dtab, sa.k2);
+                                                                        LTAB ? (const cx<T> *)tabL : (const cx<T> *)sa.dtab);
     if (valid) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -780,7 +784,6 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__r
     constexpr bool DR = MODE == S_DRIFT1;
     static_assert(MODE != S_KICK_DRIFT1, "the kick pass runs one tile per work-group");
     __shared__ cx<T> tabL[(LT && DR) ? N : 1];
-    __shared__ double k2L[(LT && MODE == S_GREEN) ? N : 1];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     cx<T> *L = lds + c * LS;
     int tile = blockIdx.x;
@@ -788,11 +791,7 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__r
     if constexpr (LT && DR) {
         for (int i = tid; i < N; i += NT) tabL[i] = ((const cx<T> *)sa.dtab)[i];
     }
-    if constexpr (LT && MODE == S_GREEN) {
-        for (int i = tid; i < N; i += NT) k2L[i] = sa.k2[i];
-    }
     const cx<T> *tab = LT ? (const cx<T> *)tabL : (const cx<T> *)sa.dtab;
-    const double *k2t = LT ? (const double *)k2L : sa.k2;
     const long stride = AXIS == 1 ? (long)rl : ps;
     // With at least TC columns per line every column of every tile lies
     // inside its row (rl >= the tile span; columns k >= kv are the half
@@ -836,7 +835,7 @@ void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__r
         __builtin_amdgcn_sched_barrier(0);   // issue the prefetch before the transform
         const int other = sa.other0 + tile / ntz, k0 = (tile % ntz) * TC, k = k0 + c;
         const bool valid = k < kv;
-        spass_compute<T, N, MODE, false, AXIS>(v, L, t, tb, sa, other, k, valid, tab, k2t);
+        spass_compute<T, N, MODE, false, AXIS>(v, L, t, tb, sa, other, k, valid, tab);
         {
             long b0, st;
             spass_addr<N, AXIS>(other, k0, rl, ps, b0, st);

@@ -175,8 +175,8 @@ int ifdm_time_passes(ifdm_ctx *ctx, const ifdm_dt *dt, int iters, float *ms_out)
  *   the forward y pass on the half spectrum (F), bit 3 its x pass (G), bit 4
  *   its inverse y pass (H); a set bit launches a resident set of
  *   work-groups that walk the tiles and prefetch the next tile's data during
- *   the current transform instead of one work-group per tile.  Default: 8
- *   (G) in FP64, 0 in mixed.
+ *   the current transform instead of one work-group per tile.  Default: 0
+ *   in both precisions.
  *   "device_loop": 1 (default) ifdm_simulate / ifdm_simulate_ex decide the
  *   time step on the device; 0 on the host after each step (the same
  *   results; a host wait per step). */

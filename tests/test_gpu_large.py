@@ -6,8 +6,8 @@
   dQI)) exactly (src/CosmoBEC_MPI_stan.cpp:121-159).  The run merges the
   half-kicks into the y passes (kick + y round trip), so every pass of the
   step's schedule runs at the production size.
-* The persistent-work-group default of the FP64 Green pass at 512^3 against
-  one work-group per tile.
+* Persistent work-groups on the FP64 Green pass at 512^3 against the
+  default of one work-group per tile.
 * The mixed path against the FP64 path from the 512^3 Zel'dovich IC (box 20
   Mpc/h, g at 1 % of gravity), over the 1000-step schedule and the
   reference's own 181 adaptive steps.
@@ -85,17 +85,17 @@ def test_plane_wave_kdk_steps_closed_form(n, prec):
     assert abs(st["norm"] / (amp * amp * n ** 3) - 1) < (1e-12 if prec == 0 else 1e-6)
 
 
-def test_persistent_green_default_512():
-    """The FP64 default launches the Green pass G on resident work-groups
-    that prefetch their next tile; at 512^3 two adaptive steps agree with
-    one-tile-per-work-group launches (option "persistent" = 0) to round-off."""
+def test_persistent_green_512():
+    """The Green pass G on resident work-groups that prefetch their next tile
+    (option "persistent" = 8) agrees at 512^3 over two adaptive steps with
+    the default one-tile-per-work-group launches to round-off."""
     from tests.fields import smooth_psi
     F, O = _mods()
     n = 512
     cfg = F.Config.from_cli(2.5e-24, 1e-58, 10.0, n)
     psi = smooth_psi(n, 64, 5, cfg.mfdm_solar)
     outs = []
-    for mask in (None, 0):
+    for mask in (None, 8):
         _say(f"[persistent 512] mask={mask}")
         with F.Solver(cfg) as s:
             if mask is not None:
