@@ -115,7 +115,7 @@ struct Ops {
             int rc;
             if ((rc = prof_begin(c))) return rc;
             hipLaunchKernelGGL((k_c2r_half<T, N>), dim3(nb), dim3(z_lines<T, N>() * P), 0, c->stream,
-                               (const cx<T> *)c->shat, (T *)c->phi, (const cx<T> *)c->tw, (const cx<double> *)c->tw64, za, c->part_phi);
+                               (const cx<T> *)c->shat, (T *)c->phi, (const cx<T> *)c->tw, za, c->part_phi);
             LAUNCHCHK(c);
             return prof_end(c);
         }

@@ -389,7 +389,7 @@ template <typename T, int N> constexpr int c2r_lines() {
 
 template <typename T, int N>
 __global__ __launch_bounds__((z_lines<T, N>() * (N / SchedT<T, N>::E)))
-void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, ZArgs za,
+void k_c2r_half(const cx<T> *__restrict__ shat, T *__restrict__ phi, const cx<T> *__restrict__ tw, ZArgs za,
                 double *__restrict__ part) {
     constexpr int H = N / 2, E = SchedT<T, H>::E, P = H / E, LZ = c2r_lines<T, N>(), LS = lds_line<H, pad_shift_t<T, H>()>();
     constexpr int HS = hs_of(N), NT = LZ * P;
