@@ -133,11 +133,15 @@ def timed_adaptive(F, cfg, psi, warmup: int, steps: int, prof_steps: int, keep_s
     around every pass for the per-pass table.  Returns (steps/s, seconds, avg
     ms per pass from the last phase, schedule of the warm-up + timed steps,
     psi and phi after them (or None), a, dominant pass index, its avg ms over
-    the timed steps)."""
+    the timed steps, the host copy times of psi: set_psi (host -> HBM, the
+    second of two calls) and get_psi (HBM -> host, after the run))."""
     s = F.Solver(cfg)
     if persistent >= 0:
         s.set_option("persistent", persistent)
     s.set_psi(psi)
+    t0 = time.perf_counter()
+    s.set_psi(psi)
+    t_set = time.perf_counter() - t0
     # one run in pieces (ifdm_simulate_ex): the entry Poisson solve belongs
     # to the first warm-up piece, each later piece resumes the run, so the
     # timed piece holds exactly `steps` steps of the run's steady state
@@ -166,9 +170,25 @@ def timed_adaptive(F, cfg, psi, warmup: int, steps: int, prof_steps: int, keep_s
     s.simulate_native(a, prof_steps)
     ms_sum, cnt = s.get_profile()
     s.set_profiling(False)
+    t0 = time.perf_counter()
+    s.get_psi()
+    t_get = time.perf_counter() - t0
     s.close()
     avg = [m / max(cnt, 1) for m in ms_sum]
-    return steps / el, el, avg, sched_w + sched_t, psi_out, phi_out, a, dom, dom_ms
+    return steps / el, el, avg, sched_w + sched_t, psi_out, phi_out, a, dom, dom_ms, (t_set, t_get)
+
+
+def host_io(t, step_s, n):
+    """psi's host copies (complex128 host arrays in the reference layout, the
+    ABI's set_psi / get_psi) beside the step time: the rate of a run of k
+    steps that starts and ends with psi in host memory.  Not the headline
+    value, which keeps psi resident in HBM."""
+    t_set, t_get = t
+    nb = 16 * n ** 3
+    return {"bytes_each_way": nb, "set_psi_s": round(t_set, 4), "get_psi_s": round(t_get, 4),
+            "set_psi_GBs": round(nb / t_set / 1e9, 2), "get_psi_GBs": round(nb / t_get / 1e9, 2),
+            "steps_per_s_incl_host_copies": {str(k): round(k / (t_set + k * step_s + t_get), 3)
+                                             for k in (1, 100, 1000)}}
 
 
 def _phase_errors(psi, psi64, w):
@@ -432,7 +452,7 @@ def main():
     cfg = F.Config.from_cli(2.5e-24, g_in, xmax_in, n, plane_pad=args.plane_pad)
     log(f"[bench] g = {g_in:.4e} eV cm^3")
 
-    sps, el, avg, sched, psi64, phi64, a, dom, dom_ms = timed_adaptive(F, cfg, psi, args.warmup, args.steps,
+    sps, el, avg, sched, psi64, phi64, a, dom, dom_ms, hio = timed_adaptive(F, cfg, psi, args.warmup, args.steps,
                                                                        args.prof_steps,
                                                                        not (args.no_mixed or args.no_mixed_check),
                                                                        args.persistent)
@@ -460,11 +480,12 @@ def main():
                      "cell_updates_per_s": round(sps * n ** 3, 1),
                      "stream_copy_GBs": round(copy_gbs, 1)},
         "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avg)},
+        "host_io": host_io(hio, el / args.steps, n),
     }
 
     if not args.no_mixed:
         cfgm = F.Config.from_cli(2.5e-24, g_in, xmax_in, n, precision=F.MIXED, plane_pad=args.plane_pad)
-        spsm, elm, avgm, _schm, _p, _f, am, domm, domm_ms = timed_adaptive(F, cfgm, psi, args.warmup, args.steps,
+        spsm, elm, avgm, _schm, _p, _f, am, domm, domm_ms, hiom = timed_adaptive(F, cfgm, psi, args.warmup, args.steps,
                                                                           args.prof_steps, False, args.persistent)
         log(f"[bench] mixed: {args.steps} steps in {elm:.3f} s ({spsm:.2f} steps/s)")
         pbm = pass_bytes(n, 8, 4)
@@ -475,6 +496,7 @@ def main():
             "step_hbm_frac_of_8TBs": round(pencil_model_bytes(n, 8) * spsm / 1e9 / HBM_PEAK_GBS, 4),
             "bytes_moved_frac_of_8TBs": round(sum(pbm) * spsm / 1e9 / HBM_PEAK_GBS, 4),
             "passes_ms": {k: round(v, 4) for k, v in zip(PASS_NAMES, avgm)},
+            "host_io": host_io(hiom, elm / args.steps, n),
         }
     if not args.no_mixed and not args.no_mixed_check:
         # error: the mixed path replays the FP64 run's own adaptive schedule

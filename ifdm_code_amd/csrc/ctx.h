@@ -3,7 +3,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ifdm.h"
@@ -123,6 +126,34 @@ static inline hipError_t plane_copy(const ifdm_ctx *c, void *field, void *dense,
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+// Host-side element loops over whole fields (the precision conversions of
+// the host copies: 2^28 elements at 512^3): fn(lo, hi, k) over
+// IFDM_HOST_CHUNKS fixed pieces of [0, n), k the piece's index, on up to 16
+// threads.  The pieces do not depend on the thread count, so per-piece
+// partial results combined in piece order give the same bits on any host.
+// Runs serially for small n or when no thread can be started.
+static constexpr int IFDM_HOST_CHUNKS = 64;
+template <typename F> static void host_chunks(size_t n, F fn) {
+    auto piece = [&](int k) {
+        fn(n * (size_t)k / IFDM_HOST_CHUNKS, n * (size_t)(k + 1) / IFDM_HOST_CHUNKS, k);
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nt = (int)std::min<unsigned>(16u, hw ? hw : 1u);
+    std::vector<std::thread> th;
+    if (nt > 1 && n >= ((size_t)1 << 20)) {
+        try {
+            for (int t = 1; t < nt; ++t)
+                th.emplace_back([&piece, t, nt] { for (int k = t; k < IFDM_HOST_CHUNKS; k += nt) piece(k); });
+        } catch (...) {
+            // fewer threads: the pieces of the missing ones run below
+        }
+    }
+    const int started = (int)th.size() + 1;
+    for (int k = 0; k < IFDM_HOST_CHUNKS; ++k)
+        if (k % nt == 0 || k % nt >= started) piece(k);
+    for (auto &x : th) x.join();
 }
 
 static constexpr int IFDM_EV_POOL = 4096;

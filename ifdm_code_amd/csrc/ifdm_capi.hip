@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -650,14 +651,23 @@ int ifdm_set_psi(ifdm_ctx *c, const double *psi) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return IFDM_OK;
     }
-    std::vector<float> tmp(2 * (size_t)c->n3);
+    const size_t ne = 2 * (size_t)c->n3;
+    std::unique_ptr<float[]> tmp(new (std::nothrow) float[ne]);
+    if (!tmp) { c->err = "host staging buffer"; return IFDM_E_NOMEM; }
+    double part[IFDM_HOST_CHUNKS];
+    const int sc = c->psi_scale;
+    host_chunks(ne, [&](size_t lo, size_t hi, int k) {
+        double s = 0.0;
+        for (size_t i = lo; i < hi; ++i) {
+            tmp[i] = (float)std::ldexp(psi[i], -sc);
+            s += psi[i] * psi[i];
+        }
+        part[k] = s;
+    });
     double norm = 0.0;
-    for (size_t i = 0; i < 2 * (size_t)c->n3; ++i) {
-        tmp[i] = (float)std::ldexp(psi[i], -c->psi_scale);
-        norm += psi[i] * psi[i];
-    }
+    for (int k = 0; k < IFDM_HOST_CHUNKS; ++k) norm += part[k];
     c->rho_mean = c->ms * norm / (double)c->n3;
-    HIPCHK(c, plane_copy(c, c->psi, tmp.data(), 2 * sizeof(float), c->n, true, hipMemcpyHostToDevice));
+    HIPCHK(c, plane_copy(c, c->psi, tmp.get(), 2 * sizeof(float), c->n, true, hipMemcpyHostToDevice));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return IFDM_OK;
 }
@@ -672,10 +682,15 @@ int ifdm_get_psi(ifdm_ctx *c, double *psi) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return IFDM_OK;
     }
-    std::vector<float> tmp(2 * (size_t)c->n3);
-    HIPCHK(c, plane_copy(c, c->psi, tmp.data(), 2 * sizeof(float), c->n, false, hipMemcpyDeviceToHost));
+    const size_t ne = 2 * (size_t)c->n3;
+    std::unique_ptr<float[]> tmp(new (std::nothrow) float[ne]);
+    if (!tmp) { c->err = "host staging buffer"; return IFDM_E_NOMEM; }
+    HIPCHK(c, plane_copy(c, c->psi, tmp.get(), 2 * sizeof(float), c->n, false, hipMemcpyDeviceToHost));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i < tmp.size(); ++i) psi[i] = std::ldexp((double)tmp[i], c->psi_scale);
+    const int sc = c->psi_scale;
+    host_chunks(ne, [&](size_t lo, size_t hi, int) {
+        for (size_t i = lo; i < hi; ++i) psi[i] = std::ldexp((double)tmp[i], sc);
+    });
     return IFDM_OK;
 }
 
@@ -688,10 +703,14 @@ int ifdm_get_phi(ifdm_ctx *c, double *phi) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return IFDM_OK;
     }
-    std::vector<float> tmp((size_t)c->n3);
-    HIPCHK(c, plane_copy(c, c->phi, tmp.data(), sizeof(float), c->n, false, hipMemcpyDeviceToHost));
+    const size_t ne = (size_t)c->n3;
+    std::unique_ptr<float[]> tmp(new (std::nothrow) float[ne]);
+    if (!tmp) { c->err = "host staging buffer"; return IFDM_E_NOMEM; }
+    HIPCHK(c, plane_copy(c, c->phi, tmp.get(), sizeof(float), c->n, false, hipMemcpyDeviceToHost));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i < tmp.size(); ++i) phi[i] = tmp[i];
+    host_chunks(ne, [&](size_t lo, size_t hi, int) {
+        for (size_t i = lo; i < hi; ++i) phi[i] = tmp[i];
+    });
     return IFDM_OK;
 }
 
@@ -705,9 +724,13 @@ int ifdm_set_phi(ifdm_ctx *c, const double *phi) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return IFDM_OK;
     }
-    std::vector<float> tmp((size_t)c->n3);
-    for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)phi[i];
-    HIPCHK(c, plane_copy(c, c->phi, tmp.data(), sizeof(float), c->n, true, hipMemcpyHostToDevice));
+    const size_t ne = (size_t)c->n3;
+    std::unique_ptr<float[]> tmp(new (std::nothrow) float[ne]);
+    if (!tmp) { c->err = "host staging buffer"; return IFDM_E_NOMEM; }
+    host_chunks(ne, [&](size_t lo, size_t hi, int) {
+        for (size_t i = lo; i < hi; ++i) tmp[i] = (float)phi[i];
+    });
+    HIPCHK(c, plane_copy(c, c->phi, tmp.get(), sizeof(float), c->n, true, hipMemcpyHostToDevice));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return IFDM_OK;
 }
@@ -1142,18 +1165,29 @@ int ifdm_adjoint(ifdm_ctx *c, const ifdm_dt *sched, int nsteps, double *g) {
     }
     // mixed: the sweep is linear in g, so g is held as 2^-sg g in FP32 with
     // |2^-sg g| <= 1 at the start (the psi scale is folded into the kicks)
+    double gpart[IFDM_HOST_CHUNKS];
+    host_chunks(n2, [&](size_t lo, size_t hi, int k) {
+        double m = 0.0;
+        for (size_t i = lo; i < hi; ++i) m = std::max(m, std::fabs(g[i]));
+        gpart[k] = m;
+    });
     double gmax = 0.0;
-    for (size_t i = 0; i < n2; ++i) gmax = std::max(gmax, std::fabs(g[i]));
+    for (int k = 0; k < IFDM_HOST_CHUNKS; ++k) gmax = std::max(gmax, gpart[k]);
     const int sg = gmax > 0 ? std::ilogb(gmax) + 1 : 0;
-    std::vector<float> tmp(n2);
-    for (size_t i = 0; i < n2; ++i) tmp[i] = (float)std::ldexp(g[i], -sg);
-    HIPCHK(c, plane_copy(c, c->adj_g, tmp.data(), 2 * sizeof(float), c->n, true, hipMemcpyHostToDevice));
+    std::unique_ptr<float[]> tmp(new (std::nothrow) float[n2]);
+    if (!tmp) { c->err = "host staging buffer"; return IFDM_E_NOMEM; }
+    host_chunks(n2, [&](size_t lo, size_t hi, int) {
+        for (size_t i = lo; i < hi; ++i) tmp[i] = (float)std::ldexp(g[i], -sg);
+    });
+    HIPCHK(c, plane_copy(c, c->adj_g, tmp.get(), 2 * sizeof(float), c->n, true, hipMemcpyHostToDevice));
     if ((rc = adjoint_sweep(c, sched, nsteps))) return rc;
-    HIPCHK(c, plane_copy(c, c->adj_g, tmp.data(), 2 * sizeof(float), c->n, false, hipMemcpyDeviceToHost));
+    HIPCHK(c, plane_copy(c, c->adj_g, tmp.get(), 2 * sizeof(float), c->n, false, hipMemcpyDeviceToHost));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     // back to physical units: g scale 2^sg, and the psi scale of the kicked
     // field enters g = e^{i theta} g' + 2 psi A s linearly through psi and s
-    for (size_t i = 0; i < n2; ++i) g[i] = std::ldexp((double)tmp[i], sg);
+    host_chunks(n2, [&](size_t lo, size_t hi, int) {
+        for (size_t i = lo; i < hi; ++i) g[i] = std::ldexp((double)tmp[i], sg);
+    });
     return IFDM_OK;
 }
 
