@@ -115,15 +115,15 @@ template <typename T, int N> constexpr int z_lines() {
 // 1.88, 1.91 (1.97, 2.01) ms mixed; in FP64 they keep 128-byte rows (3.57,
 // 3.46 against 3.64, 3.65 ms narrow).  The x passes B and G take narrow
 // tiles too, grouped the same way (B 5.40 -> 4.98 ms mixed, 10.3 -> 9.6 ms
-// FP64; G 2.98 -> 2.63 ms mixed; the FP64 G pass keeps 128-byte rows: 5.67
-// ms either way, and narrow rows wrote 12 % more than its bytes); without
-// the grouping and the addressing of the current build they had measured
-// slower (B 5.8 -> 6.9 ms mixed).  At N = 512, where two 128-byte tiles
+// FP64; G 2.98 -> 2.63 ms mixed, 5.29 -> 4.41 ms FP64, where 5.67 ms either
+// way while its Green factor loaded k^2 from a table); without the grouping
+// and the addressing of the current build they had measured slower (B 5.8
+// -> 6.9 ms mixed).  At N = 512, where two 128-byte tiles
 // already share a CU, narrow x tiles are slower (FP64 B 0.83 -> 1.05 ms,
 // mixed B 0.44 -> 0.46, G 0.248 -> 0.256).
 template <typename T, int N, bool WIDE = false, int AXIS = 1, int MODE = -1> constexpr bool s_narrow() {
     return N == 1024 && !WIDE && ((AXIS == 1 && (sizeof(T) == 4 || MODE == S_KICK_DRIFT1 || MODE == S_DRIFT1)) ||
-                                  (AXIS == 0 && (MODE == S_DRIFT1 || (MODE == S_GREEN && sizeof(T) == 4))));
+                                  (AXIS == 0 && (MODE == S_DRIFT1 || MODE == S_GREEN)));
 }
 
 // Schedule variant of a strided pass's transforms (Sched): radix-16 stages
