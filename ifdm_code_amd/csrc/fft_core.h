@@ -78,8 +78,11 @@ template <typename T, int N, int V> constexpr int ns_before(int s) {
 // LDS position of point p of a line: one pad slot per 2^SH points.  SH = 4
 // where the first stage has radix 16 (its scatter writes 16 consecutive
 // points per thread: lanes then start 17 slots apart, which covers the
-// banks once per 16 lanes; 18 apart would pair them), else 3.
-template <typename T, int N, int V = 0> constexpr int pad_shift_t() { return SchedT<T, N, V>::r[0] >= 16 ? 4 : 3; }
+// banks once per 16 lanes; 18 apart would pair them), else 3.  Variant
+// V = 2 keeps the radices of V = 0 with SH = 4 (a shorter line buffer).
+template <typename T, int N, int V = 0> constexpr int pad_shift_t() {
+    return (V == 2 || SchedT<T, N, V>::r[0] >= 16) ? 4 : 3;
+}
 template <int SH> __device__ __forceinline__ int lpos(int p) { return p + (p >> SH); }
 // LDS line stride in elements of the z passes: congruent to 2 mod 16 so that
 // the lines of a wavefront holding several short lines fall on different

@@ -132,7 +132,13 @@ template <typename T, int N, bool WIDE = false, int AXIS = 1, int MODE = -1> con
 // G 3.13 -> 2.84, F 1.86 -> 1.85, H 1.91 -> 1.88; the kick pass A, capped
 // at 128 VGPRs, 7.43 -> 7.72, so it keeps radices <= 8 (as do the z passes,
 // whose pass C took 205 VGPRs with radix 16: 4.84 -> 5.09).
-template <int MODE> constexpr int sp_sched() { return MODE == S_KICK_DRIFT1 ? 0 : 1; }
+// The FP64 drift passes at N = 1024 take variant 2 (radices <= 8, one LDS
+// pad slot per 16 points): their narrow tiles' exchange buffer shrinks from
+// 74 to 70 KB, so the drift table fits beside it with two work-groups per CU.
+template <typename T, int N, int MODE> constexpr int sp_sched() {
+    return (sizeof(T) == 8 && N == 1024 && (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1)) ? 2
+                                                                                      : (MODE == S_KICK_DRIFT1 ? 0 : 1);
+}
 
 // z-columns per strided-pass tile: 128-byte row segments (8 complex
 // doubles, 16 complex floats; 64-byte ones for narrow tiles); WIDE doubles
@@ -144,7 +150,7 @@ template <typename T, int N, bool WIDE = false, int MODE = -1, int AXIS = 1> con
     constexpr int want = (WIDE && sizeof(cx<T>) == 16) ? 2 * base : base;
     constexpr int cap = 1024 / (N / SchedT<T, N>::E);
     // and the exchange buffer within the 160 KB of LDS
-    constexpr int lcap = 163840 / (lds_col_line<N, narrow, pad_shift_t<T, N, sp_sched<MODE>()>()>() * (int)sizeof(cx<T>));
+    constexpr int lcap = 163840 / (lds_col_line<N, narrow, pad_shift_t<T, N, sp_sched<T, N, MODE>()>()>() * (int)sizeof(cx<T>));
     int c = want < cap ? want : cap;
     while (c > lcap) c /= 2;
     return c;
@@ -662,7 +668,7 @@ template <typename T, int N, int AXIS, int MODE, bool WIDE>
 __global__ __launch_bounds__((s_cols<T, N, WIDE, MODE, AXIS>() * (N / SchedT<T, N>::E)), (sp_wpe<T, MODE, N, WIDE, AXIS>()))
 void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa) {
     constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, WIDE, MODE, AXIS>();
-    constexpr int SV = sp_sched<MODE>(), LS = lds_col_line<N, s_narrow<T, N, WIDE, AXIS, MODE>(), pad_shift_t<T, N, SV>()>();
+    constexpr int SV = sp_sched<T, N, MODE>(), LS = lds_col_line<N, s_narrow<T, N, WIDE, AXIS, MODE>(), pad_shift_t<T, N, SV>()>();
     __shared__ cx<T> lds[TC * LS];
     const int tid = threadIdx.x, c = tid % TC, t = tid / TC;
     int bx = blockIdx.x, by = blockIdx.y;
@@ -672,16 +678,17 @@ void k_spass(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__rest
     } else if constexpr (MODE == S_KICK_DRIFT1 && (N / TC) % 16 == 0) {
         xcd_pair(bx, by);
     }
-    // The drift passes at N = 512, and the complex-float ones at N = 1024,
-    // read the drift table from LDS.  From global memory its loads, issued
-    // after the forward transform, cost an L2 round trip per tile, and under
-    // the 128-VGPR cap of the 1024-point complex-float tiles each of the 16
-    // was awaited before the next.  Narrow tiles and complex doubles keep
-    // entries 0..N/2 (see spass_compute): two work-groups per CU leave no
-    // room for all N.  Measured at 512^3 (ms, global -> LDS table): mixed
-    // A 0.60 -> 0.54, B 0.48 -> 0.44; FP64 A 0.96 -> 0.95, B 0.88 -> 0.83.
+    // The drift passes read the drift table from LDS (at N = 1024 in FP64
+    // through schedule variant 2, whose shorter line buffer leaves the room).
+    // From global memory its loads, issued after the forward transform, cost
+    // an L2 round trip per tile, and under the 128-VGPR cap of the 1024-point
+    // complex-float tiles each of the 16 was awaited before the next.  Narrow
+    // tiles and complex doubles keep entries 0..N/2 (see spass_compute): two
+    // work-groups per CU leave no room for all N.  Measured (ms, global ->
+    // LDS table) at 512^3: mixed A 0.60 -> 0.54, B 0.48 -> 0.44; FP64 A 0.96
+    // -> 0.95, B 0.88 -> 0.83; at 1024^3 FP64: A 9.77 -> 9.19, B 9.54 -> 8.98.
     // The forward transform's exchanges order these writes before the reads.
-    constexpr bool LTAB = (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1) && (N == 512 || (N == 1024 && sizeof(T) == 4));
+    constexpr bool LTAB = (MODE == S_DRIFT1 || MODE == S_KICK_DRIFT1) && (N == 512 || (N == 1024 && (sizeof(T) == 4 || SV == 2)));
     constexpr bool HALF = LTAB && (s_narrow<T, N, WIDE, AXIS, MODE>() || sizeof(T) == 8);
     constexpr int TABN = LTAB ? (HALF ? N / 2 + 1 : N) : 1;
     __shared__ cx<T> tabL[TABN];
@@ -772,7 +779,7 @@ template <typename T, int N, int AXIS, int MODE>
 __global__ __launch_bounds__((s_cols<T, N, false, MODE, AXIS>() * (N / SchedT<T, N>::E)))
 void k_spass_pp(cx<T> *__restrict__ a, int rl, long ps, int kv, const cx<T> *__restrict__ tw, const cx<double> *__restrict__ tw64, SArgs sa, int ntz,
                 int ntiles) {
-    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, false, MODE, AXIS>(), SV = sp_sched<MODE>();
+    constexpr int E = SchedT<T, N>::E, P = N / E, TC = s_cols<T, N, false, MODE, AXIS>(), SV = sp_sched<T, N, MODE>();
     constexpr int LS = lds_col_line<N, false, pad_shift_t<T, N, SV>()>();
     constexpr int NT = TC * P;
     __shared__ cx<T> lds[TC * LS];
