@@ -88,7 +88,10 @@ const char *ifdm_last_error(const ifdm_ctx *ctx);
 const char *ifdm_create_error(void);
 
 /* psi in / out, host complex128 buffers of N^3 (re, im) pairs in the
- * reference layout (the opr.wfc vector, src/struct3D.h:102). */
+ * reference layout (the opr.wfc vector, src/struct3D.h:102).  In mixed
+ * mode the precision conversion runs on up to 16 host threads that the
+ * call starts and joins before it returns (also in ifdm_set_phi,
+ * ifdm_get_phi and ifdm_adjoint). */
 int ifdm_set_psi(ifdm_ctx *ctx, const double *psi_c128);
 int ifdm_get_psi(ifdm_ctx *ctx, double *psi_c128);
 /* phi out, N^3 doubles (opr.phi with its pad columns dropped). */
