@@ -627,3 +627,27 @@ def test_live_profiling_counts_every_pass(loop):
         assert cnt2 == 3 and ms2[3] > 0.0
         assert all(x == 0.0 for i, x in enumerate(ms2) if i != 3), ms2
         assert np.array_equal(s.get_psi(), ref)
+
+
+@pytest.mark.parametrize("n", [16, 128])
+def test_mixed_host_copies_round_once(n):
+    """Mixed mode stores 2^-s psi as complex64 and phi as floats:
+    ifdm_set_psi then ifdm_get_psi return 2^s float(2^-s psi), bit for bit,
+    likewise ifdm_set_phi / ifdm_get_phi float(phi).  At n = 128 (2^22 values)
+    the conversions run on the host threads in 64 pieces; at n = 16 serially."""
+    F, _O = _mods()
+    cfg = F.Config.from_cli(2.5e-24, 0.0, 5.0, n, precision=F.MIXED)
+    psi = make_psi(n, 11, cfg.mfdm_solar)
+    from tests.fields import grf
+    phi = grf(n, 12, rms=3.0e3)
+    _say(f"[host copies] n={n}")
+    with F.Solver(cfg) as s:
+        s.set_psi(psi)
+        back = s.get_psi()
+        s.set_phi(phi)
+        phi_back = s.get_phi()
+    # the library's scale exponent: round(log2(rhoc / m_s) / 2), half away from zero
+    sc = math.floor(0.5 * math.log2(cfg.rhoc / cfg.mfdm_solar) + 0.5)
+    want = np.ldexp(np.ldexp(psi.view(np.float64), -sc).astype(np.float32).astype(np.float64), sc)
+    assert np.array_equal(back.view(np.float64), want)
+    assert np.array_equal(phi_back, phi.astype(np.float32).astype(np.float64))
